@@ -1,0 +1,217 @@
+/*
+ * wave-tool water-displacement stage for MI355X (gfx950) — C ABI.
+ *
+ * The reference evaluates this stage in an OpenGL vertex shader
+ * (assets/shaders/water-grid.vert) driven by the water block of
+ * RenderEngine::render (src/render-engine.cpp:796-1123). That block's host half
+ * (projector and uniforms) and device half (per-vertex Gerstner sum, heightmap
+ * bump, normals) are replaced by the entry points below. Each declaration names
+ * the reference interface it replaces (paths relative to the reference root).
+ *
+ * Conventions
+ *  - Plain C types only; device pointers are plain pointers, streams are
+ *    `void*` holding a hipStream_t (NULL = the default stream).
+ *  - Every entry point returns a wt_status; no exception crosses the ABI.
+ *    wt_last_error() returns the message of the last failing call on a context.
+ *  - Calls on one context are stream-ordered; one context per host thread.
+ *  - Vertex output layout is the reference's MeshObject attribute layout
+ *    (src/mesh-object.h:74-75, src/render-engine.cpp:1175-1191): tightly packed
+ *    float3 arrays (12-byte stride), vertex id = row * gridLength + column with
+ *    row 0 the bottom edge of the grid (water-grid.vert:115-125).
+ */
+#ifndef WT_H_
+#define WT_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WT_ABI_VERSION 1
+
+/* Largest wave count one call accepts (the reference fixes 4,
+ * water-grid.vert:35 and src/render-engine.h:132). */
+#define WT_MAX_WAVES 1024
+
+/* Algorithmic work units used for roofline accounting (not executed counts).
+ * Forward: 26 FP32 FLOP per vertex-wave (FMA = 2, sin and cos = 1 each):
+ * phase 5, sin+cos 2, position 7, analytic partials 12. Bytes: HBM writes of
+ * position + normal (24 B per vertex) or with tangent + bitangent (48 B).
+ * Adjoint: 50 FLOP per vertex-wave-snapshot per iteration. */
+#define WT_FLOP_PER_VERTEX_WAVE 26
+#define WT_BYTES_PER_VERTEX_PN 24
+#define WT_BYTES_PER_VERTEX_PNTB 48
+#define WT_ADJOINT_FLOP_PER_VERTEX_WAVE_SNAPSHOT 50
+
+typedef enum wt_status {
+  WT_OK = 0,
+  WT_ERR_INVALID_ARG = 1,
+  WT_ERR_NO_INTERSECTION = 2, /* frustum misses the displaceable slab (render-engine.cpp:894) */
+  WT_ERR_HIP = 3,
+  WT_ERR_OOM = 4,
+  WT_ERR_UNSUPPORTED = 5,
+  WT_ERR_DEGENERATE = 6 /* a projector ray parallel to y = 0 (render-engine.cpp:934-938, 1034 asserts) */
+} wt_status;
+
+typedef struct wt_ctx wt_ctx;
+typedef struct wt_batch wt_batch;
+typedef struct wt_graph wt_graph;
+typedef struct wt_fit wt_fit;
+
+/* One Gerstner wave: the GLSL uniform struct GerstnerWave
+ * (water-grid.vert:45-51) in its member order, plus an appended phase offset
+ * phi0 added to the phase (zero reproduces the reference). steepness_Q_i is
+ * the already-normalised Q_i the shader receives (see wt_normalize_steepness).
+ * Passed to the kernels as a device array. */
+typedef struct wt_gerstner_wave {
+  float amplitude_A;
+  float frequency_w;
+  float phaseConstant_phi;
+  float steepness_Q_i;
+  float xzDirection_D[2];
+  float phaseOffset_phi0;
+  float reserved;
+} wt_gerstner_wave;
+
+/* Camera state of src/camera.h:60-72 that the projector reads. The camera is
+ * built with the reference defaults (forward (0,0,-1), up (0,1,0),
+ * src/camera.cpp:40-45) and then setRotation(yaw_deg, pitch_deg) is applied. */
+typedef struct wt_camera {
+  float fov_deg;
+  float aspect;
+  float near_clip;
+  float far_clip;
+  float position[3];
+  float yaw_deg;
+  float pitch_deg;
+} wt_camera;
+
+typedef enum wt_wrap { WT_WRAP_REPEAT = 0, WT_WRAP_MIRRORED_REPEAT = 1 } wt_wrap;
+typedef enum wt_texel_format { WT_TEXEL_R8_UNORM = 0, WT_TEXEL_R32F = 1 } wt_texel_format;
+
+/* Heightmap sampler state (src/texture.cpp:54-72): bilinear GL_LINEAR
+ * filtering with texel centres at (i + 1/2) / size, wrap REPEAT or
+ * MIRRORED_REPEAT (the reference uses MIRRORED_REPEAT, texture.cpp:63-64).
+ * texels: device pointer, row-major, row 0 = texture t in [0, 1/height)
+ * (GL memory order, i.e. after the loader's vertical flip,
+ * render-engine.cpp:1310). R8_UNORM holds the .r channel the shader reads
+ * (water-grid.vert:107) as one byte per texel. */
+typedef struct wt_heightmap {
+  const void* texels;
+  uint32_t width;
+  uint32_t height;
+  uint32_t format; /* wt_texel_format */
+  uint32_t wrap;   /* wt_wrap */
+} wt_heightmap;
+
+/* The water program's uniform set (water-grid.vert:37-64, uploaded at
+ * render-engine.cpp:1057-1110). Corners are world positions on y = 0 in the
+ * order of waterGridCornerPoints (render-engine.cpp:1012-1015): [0] bottom-left,
+ * [1] top-left, [2] bottom-right, [3] top-right, each (x, y, z, w); they are
+ * double so the FP64 phase reduction sees exact grid positions. The scroll
+ * velocity moves the heightmap sample point by -v*t (zero in the reference). */
+typedef struct wt_water_params {
+  double corners[4][4];
+  float cameraPosition[3];
+  uint32_t gridLength;
+  float heightmapSampleScale;
+  float heightmapDisplacementScale;
+  float verticalBounceWaveDisplacement;
+  float heightmapScrollVelocity[2];
+  float reserved;
+  double waveAnimationTimeInSeconds;
+} wt_water_params;
+
+/* Caller-owned device output arrays, each gridLength^2 float3 (12-byte stride).
+ * position and normal are required; tangent and bitangent may be NULL. */
+typedef struct wt_vertex_buffers {
+  float* position;
+  float* normal;
+  float* tangent;
+  float* bitangent;
+} wt_vertex_buffers;
+
+/* Kernel launch shape for the forward kernels; 0 in a field means "autotuned
+ * or default". variant: 1 = VALU accumulation, 2 = MFMA 4x4x1 accumulation. */
+typedef struct wt_launch_config {
+  uint32_t variant;
+  uint32_t steps_per_lane; /* columns each lane advances by the sin/cos recurrence: 4, 8 or 16 */
+  uint32_t lanes_per_row;  /* lanes of a 64-wide wavefront sharing one grid row: 64, 32, 16 or 8 */
+  uint32_t wave_chunk;     /* waves staged in LDS per pass: 32, 64 or 128 */
+} wt_launch_config;
+
+/* ---------------------------------------------------------------- context */
+int wt_abi_version(void);
+const char* wt_status_string(wt_status s);
+/* Binds to the HIP device current on the calling thread. */
+wt_status wt_create(wt_ctx** out);
+void wt_destroy(wt_ctx* ctx);
+const char* wt_last_error(const wt_ctx* ctx);
+
+/* ---------------------------------------------------------------- host helpers */
+/* Camera::updateBasisVectors / updateProjectionMat / updateViewMat
+ * (src/camera.cpp:130-146): column-major view and projection matrices and the
+ * forward vector, in double. */
+wt_status wt_camera_matrices(const wt_camera* cam, double view[16], double proj[16], double forward[3]);
+
+/* Projector: the host half of the water block, src/render-engine.cpp:797-1042
+ * (with Line/Plane/linePlaneIntersection, src/render-engine.h:53-124, and the
+ * Camera clone). Returns the four world-space grid corners in the
+ * wt_water_params.corners order. displaceable_amplitude is the slab half-height
+ * of render-engine.cpp:803 (passed explicitly; see the reference's stale
+ * TotalAmplitude, render-engine.h:156-162). Computed in double.
+ * WT_ERR_NO_INTERSECTION where the reference silently skips the water. */
+wt_status wt_projector_corners(const wt_camera* cam, float displaceable_amplitude, double corners[4][4]);
+
+/* Steepness normalisation and upload value, render-engine.cpp:1071-1073:
+ * Q_i = (w * A != 0) ? q / (w * A * normaliser_count) : 0, written into
+ * waves[i].steepness_Q_i. normaliser_count is GerstnerWave::Count() in the
+ * reference (the live wave count). */
+wt_status wt_normalize_steepness(wt_gerstner_wave* waves, const float* steepness_q, uint32_t count,
+                                 uint32_t normaliser_count);
+
+/* ---------------------------------------------------------------- forward */
+/* The water vertex program (water-grid.vert:114-193) over the whole
+ * gridLength^2 grid: bilinear corner interpolation, Gerstner sum, heightmap
+ * bump sampled at the displaced point, vertical bounce; analytic normal
+ * normalize(dP/dz x dP/dx) (heightmap term included by the chain rule) flipped
+ * toward the camera as at water-grid.vert:190, tangent normalize(dP/dx),
+ * bitangent normalize(dP/dz). Replaces the glDrawElements of
+ * render-engine.cpp:1115 plus the vertex stage it runs. d_waves: device array
+ * of wave_count waves. heightmap may be NULL (no bump). */
+wt_status wt_water_displace(wt_ctx* ctx, const wt_water_params* params, const wt_gerstner_wave* d_waves,
+                            uint32_t wave_count, const wt_heightmap* heightmap, const wt_vertex_buffers* out,
+                            void* stream);
+
+/* Overrides (or with all-zero fields, restores) the launch shape of the plain
+ * forward kernel. Returns WT_ERR_INVALID_ARG for unsupported values. */
+wt_status wt_set_launch_config(wt_ctx* ctx, const wt_launch_config* cfg);
+/* The launch shape wt_water_displace would use for this problem size. */
+wt_status wt_get_launch_config(wt_ctx* ctx, uint32_t grid_length, uint32_t wave_count, wt_launch_config* cfg);
+
+/* Times every supported launch shape for (grid_length, wave_count) on the
+ * context's device with synthetic inputs and stores the fastest in the
+ * context's table (per size bucket). best_ms (may be NULL) receives its
+ * per-launch time. Allocates and frees its own scratch buffers. */
+wt_status wt_autotune_forward(wt_ctx* ctx, uint32_t grid_length, uint32_t wave_count, int with_heightmap,
+                              wt_launch_config* best, float* best_ms);
+/* Save / load the autotune table as text (one line per size bucket). */
+wt_status wt_autotune_save(wt_ctx* ctx, const char* path);
+wt_status wt_autotune_load(wt_ctx* ctx, const char* path);
+
+/* ---------------------------------------------------------------- graphs */
+/* Stream capture of any sequence of wt_* calls into a HIP graph, so a frame
+ * loop replays with one launch (the reference's per-frame render loop,
+ * src/program.cpp:83-99). */
+wt_status wt_graph_begin(wt_ctx* ctx, void* stream);
+wt_status wt_graph_end(wt_ctx* ctx, void* stream, wt_graph** out);
+wt_status wt_graph_launch(wt_graph* graph, void* stream);
+void wt_graph_destroy(wt_graph* graph);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WT_H_ */

@@ -1,0 +1,255 @@
+"""GPU parity of the plain forward kernel (wt_water_displace) against the FP64 oracle.
+
+Tolerance: positions within 1e-4 m and normals within
+1e-4 rad for every vertex within 1 km of the camera. Normals of vertices whose
+heightmap sample lies within 2e-3 texel of a texel edge are left out of the normal
+check (the bilinear gradient is discontinuous there, so an FP32 and an FP64 sample
+point can legitimately fall in different cells); the excluded count is asserted small.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle as orc
+import wavetool_amd as wt
+from wavetool_amd import scenes
+from helpers import (POS_TOL, NRM_TOL, compare, near_texel_edge, oracle_scene)
+
+pytestmark = pytest.mark.gpu
+
+NTHREADS = min(16, os.cpu_count() or 1)
+
+
+def say(msg):
+    print("[case] " + msg, flush=True)
+
+
+def gpu_displace(torch, ctx, params, waves_np, N, texels=None, fmt=wt.WT_TEXEL_R32F, wrap=wt.WT_WRAP_REPEAT,
+                 with_tb=False, stream=None):
+    dev = torch.device("cuda")
+    V = N * N
+    waves = torch.from_numpy(np.ascontiguousarray(waves_np, dtype=np.float32).reshape(-1, 8)).to(dev)
+    bufs = [torch.full((V * 3,), float("nan"), dtype=torch.float32, device=dev) for _ in range(4 if with_tb else 2)]
+    hm = None
+    if texels is not None:
+        hm = torch.from_numpy(np.ascontiguousarray(texels)).to(dev)
+    ctx.displace(params, waves, bufs[0], bufs[1], bufs[2] if with_tb else None, bufs[3] if with_tb else None,
+                 heightmap=hm, hm_format=fmt, hm_wrap=wrap, stream=stream)
+    torch.cuda.synchronize()
+    out = dict(pos=bufs[0].cpu().numpy().reshape(V, 3), nrm=bufs[1].cpu().numpy().reshape(V, 3))
+    if with_tb:
+        out["tan"] = bufs[2].cpu().numpy().reshape(V, 3)
+        out["bit"] = bufs[3].cpu().numpy().reshape(V, 3)
+    return out
+
+
+def scene_setup(cfg_index, frame=0):
+    cfg = scenes.plain_config(cfg_index)
+    corners = wt.projector_corners(cfg["camera"], cfg["amp"])
+    waves = scenes.gen_waves(cfg["seed"], cfg["W"])
+    cam_pos = cfg["camera"][4:7]
+    t = cfg["t0"] + frame * cfg["dt"]
+    texels = scenes.value_noise(cfg["heightmap"]["seed"], cfg["heightmap"]["size"]) if cfg["heightmap"] else None
+    return cfg, corners, waves, cam_pos, t, texels
+
+
+def check_parity(torch, ctx, cfg, corners, waves, cam_pos, t, texels, with_tb, fmt=wt.WT_TEXEL_R32F,
+                 wrap=wt.WT_WRAP_REPEAT, bounce=0.0):
+    N = cfg["N"]
+    params = wt.water_params(corners, cam_pos, N, t, cfg["sample_scale"], cfg["displacement_scale"], bounce,
+                             cfg["scroll"])
+    out = gpu_displace(torch, ctx, params, waves, N, texels, fmt, wrap, with_tb)
+    sd = oracle_scene(corners, cam_pos, N, waves, t, cfg["sample_scale"], cfg["displacement_scale"], bounce,
+                      cfg["scroll"], texels, fmt, wrap)
+    ref = orc.vertices(sd, nthreads=NTHREADS, want=("pos", "nrm", "tan", "bit") if with_tb else ("pos", "nrm"))
+    edge = None
+    if texels is not None:
+        h, w = texels.shape
+        edge = near_texel_edge(ref["pos"], cfg["sample_scale"], cfg["scroll"], t, w, h)
+    res = compare(out, ref, cam_pos, edge)
+    assert np.isfinite(out["pos"]).all() and np.isfinite(out["nrm"]).all()
+    return res
+
+
+def test_config0_parity(torch_cuda, wtctx):
+    """256^2 grid, 4 waves, t = 10 s, position + N/T/B against the FP64 oracle."""
+    say("config0 256x256 W=4 t=10 with T/B")
+    cfg, corners, waves, cam_pos, t, texels = scene_setup(0)
+    res = check_parity(torch_cuda, wtctx, cfg, corners, waves, cam_pos, t, texels, True)
+    print(res, flush=True)
+    assert res["n_near"] > 0.5 * cfg["N"] ** 2
+    assert res["pos_max"] <= POS_TOL
+    assert res["nrm_max"] <= NRM_TOL
+    assert res["tan_max"] <= NRM_TOL and res["bit_max"] <= NRM_TOL
+
+
+@pytest.mark.parametrize("frame", [0, 239])
+def test_config1_frame_parity(torch_cuda, wtctx, frame):
+    """1024^2 grid, 64 waves, 512^2 FP32 value-noise heightmap (REPEAT, scrolled)."""
+    say("config1 1024x1024 W=64 heightmap frame %d" % frame)
+    cfg, corners, waves, cam_pos, t, texels = scene_setup(1, frame)
+    res = check_parity(torch_cuda, wtctx, cfg, corners, waves, cam_pos, t, texels, False)
+    print(res, flush=True)
+    assert res["pos_max"] <= POS_TOL
+    assert res["nrm_max"] <= NRM_TOL
+    assert res["n_excluded"] < 0.02 * res["n_near"]
+    assert res["n_flip"] < 1e-4 * res["n_near"]
+
+
+SHAPES = [(v, k, lc) for v in (1, 2) for k in (4, 8, 16) for lc in (64, 32, 16, 8)]
+
+
+@pytest.mark.parametrize("variant,steps,lanes", SHAPES)
+def test_launch_shapes(torch_cuda, wtctx, variant, steps, lanes):
+    """Every launch shape on a ragged size (N = 301 is no multiple of any segment
+    width) and a wave count that spans two LDS chunks (W = 70)."""
+    say("shape variant=%d steps=%d lanes=%d N=301 W=70" % (variant, steps, lanes))
+    cam = scenes.camera_tuple(height=6.0, pitch_deg=-25.0, yaw_deg=30.0)
+    corners = wt.projector_corners(cam, 2.0)
+    waves = scenes.gen_waves(11, 70)
+    texels = scenes.value_noise(5, 64)
+    cfg = dict(N=301, sample_scale=0.05, displacement_scale=0.1, scroll=(0.2, -0.1))
+    wtctx.set_launch_config(variant, steps, lanes, 64)
+    try:
+        res = check_parity(torch_cuda, wtctx, cfg, corners, waves, cam[4:7], 37.25, texels, True)
+    finally:
+        wtctx.set_launch_config()
+    print(res, flush=True)
+    assert res["pos_max"] <= POS_TOL and res["nrm_max"] <= NRM_TOL
+
+
+def test_small_and_empty(torch_cuda, wtctx):
+    """N = 2 and N = 3 grids, zero waves and one wave."""
+    cam = scenes.camera_tuple()
+    corners = wt.projector_corners(cam, 2.0)
+    for N in (2, 3, 65):
+        for W in (0, 1, 5):
+            say("small N=%d W=%d" % (N, W))
+            waves = scenes.gen_waves(3, W) if W else np.zeros((0, 8), np.float32)
+            cfg = dict(N=N, sample_scale=0.0, displacement_scale=0.0, scroll=(0.0, 0.0))
+            res = check_parity(torch_cuda, wtctx, cfg, corners, waves if W else np.zeros((1, 8), np.float32)[:0],
+                               cam[4:7], 1.5, None, True)
+            assert res["pos_max"] <= POS_TOL and res["nrm_max"] <= NRM_TOL
+
+
+def test_zero_waves_is_flat(torch_cuda, wtctx):
+    """W = 0 and no heightmap: positions are the bilinear grid points on y = bounce and
+    every normal is +y (camera above the plane)."""
+    say("zero waves flat grid")
+    cam = scenes.camera_tuple()
+    corners = wt.projector_corners(cam, 2.0)
+    N = 33
+    params = wt.water_params(corners, cam[4:7], N, 0.0, bounce=0.25)
+    out = gpu_displace(torch_cuda, wtctx, params, np.zeros((0, 8), np.float32), N)
+    u = np.arange(N) / (N - 1)
+    BL, TL, BR, TR = corners
+    g = (BL[None, None, :] * (1 - u)[None, :, None] + BR[None, None, :] * u[None, :, None]) * (1 - u)[:, None, None] + \
+        (TL[None, None, :] * (1 - u)[None, :, None] + TR[None, None, :] * u[None, :, None]) * u[:, None, None]
+    g = g.reshape(-1, 4)
+    assert np.abs(out["pos"][:, 0] - g[:, 0]).max() < 1e-3 * max(1.0, np.abs(g[:, 0]).max() * 1e-3)
+    assert np.abs(out["pos"][:, 1] - 0.25).max() == 0.0
+    assert np.abs(out["nrm"] - np.array([0, 1, 0], np.float32)).max() < 1e-7
+
+
+def test_r8_mirrored_heightmap_and_bounce(torch_cuda, wtctx):
+    """The reference's sampler state: 8-bit texels (unorm), GL_MIRRORED_REPEAT, with
+    the reference default scales (s_h = 0.02, s_d = 1.0) and a vertical bounce."""
+    say("R8 mirrored heightmap, bounce")
+    cam = scenes.camera_tuple(height=4.0, pitch_deg=-10.0)
+    corners = wt.projector_corners(cam, 3.1)
+    waves = scenes.gen_waves(21, 4)
+    rng = np.random.default_rng(0)
+    texels = rng.integers(0, 256, size=(48, 40), dtype=np.uint8)
+    cfg = dict(N=257, sample_scale=0.02, displacement_scale=1.0, scroll=(0.0, 0.0))
+    res = check_parity(torch_cuda, wtctx, cfg, corners, waves, cam[4:7], 5.0, texels, True,
+                       fmt=wt.WT_TEXEL_R8_UNORM, wrap=wt.WT_WRAP_MIRRORED_REPEAT, bounce=0.0618)
+    print(res, flush=True)
+    assert res["pos_max"] <= POS_TOL and res["nrm_max"] <= NRM_TOL
+
+
+def test_long_time_far_field(torch_cuda, wtctx):
+    """t = 3600 s and corners up to ~1.5 km: the FP64 phase reduction keeps FP32 parity."""
+    say("t=3600 far field")
+    cam = scenes.camera_tuple(height=30.0, pitch_deg=-5.0, yaw_deg=140.0)
+    corners = wt.projector_corners(cam, 2.0)
+    waves = scenes.gen_waves(31, 48)
+    cfg = dict(N=512, sample_scale=0.0, displacement_scale=0.0, scroll=(0.0, 0.0))
+    res = check_parity(torch_cuda, wtctx, cfg, corners, waves, cam[4:7], 3599.75, None, False)
+    print(res, flush=True)
+    assert res["pos_max"] <= POS_TOL and res["nrm_max"] <= NRM_TOL
+
+
+def test_graph_replay_matches_eager(torch_cuda, wtctx):
+    """A 12-frame loop captured as a HIP graph, replayed 3 times with eager work and
+    synchronisation between replays; every replay equals the eager frames bit-exactly."""
+    say("graph replay 12 frames")
+    torch = torch_cuda
+    cfg, corners, waves, cam_pos, t, texels = scene_setup(1)
+    N, F = 192, 12
+    dev = torch.device("cuda")
+    wv = torch.from_numpy(waves).to(dev)
+    hm = torch.from_numpy(texels).to(dev)
+    pos = torch.zeros((F, N * N * 3), dtype=torch.float32, device=dev)
+    nrm = torch.zeros_like(pos)
+    params = [wt.water_params(corners, cam_pos, N, 10.0 + k / 60.0, 0.02, 0.05, 0.0, (0.3, 0.0)) for k in range(F)]
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        for k in range(F):
+            wtctx.displace(params[k], wv, pos[k], nrm[k], heightmap=hm, stream=s)
+    s.synchronize()
+    eager_p, eager_n = pos.clone(), nrm.clone()
+    pos.zero_()
+    nrm.zero_()
+    torch.cuda.synchronize()
+    wtctx.graph_begin(s)
+    for k in range(F):
+        wtctx.displace(params[k], wv, pos[k], nrm[k], heightmap=hm, stream=s)
+    g = wtctx.graph_end(s)
+    for rep in range(3):
+        pos.fill_(float("nan"))
+        nrm.fill_(float("nan"))
+        torch.cuda.synchronize()
+        g.launch(s)
+        s.synchronize()
+        assert torch.equal(pos, eager_p), rep
+        assert torch.equal(nrm, eager_n), rep
+        _ = (pos * 2.0).sum().item()  # eager work between replays
+    g.close()
+
+
+def test_invalid_arguments(torch_cuda, wtctx):
+    say("invalid arguments")
+    torch = torch_cuda
+    dev = torch.device("cuda")
+    cam = scenes.camera_tuple()
+    corners = wt.projector_corners(cam, 2.0)
+    waves = torch.from_numpy(scenes.gen_waves(1, 4)).to(dev)
+    N = 16
+    pos = torch.zeros(N * N * 3, device=dev)
+    nrm = torch.zeros(N * N * 3, device=dev)
+    bad = wt.water_params(corners, cam[4:7], 1, 0.0)
+    with pytest.raises(wt.WtError) as e:
+        wtctx.displace(bad, waves, pos, nrm)
+    assert e.value.status == wt.WT_ERR_INVALID_ARG
+    nan_t = wt.water_params(corners, cam[4:7], N, float("nan"))
+    with pytest.raises(wt.WtError) as e:
+        wtctx.displace(nan_t, waves, pos, nrm)
+    assert e.value.status == wt.WT_ERR_INVALID_ARG
+    host = torch.zeros(N * N * 3)  # host memory is rejected, not dereferenced
+    ok = wt.water_params(corners, cam[4:7], N, 0.0)
+    with pytest.raises(wt.WtError) as e:
+        wtctx.displace(ok, waves, host, nrm)
+    assert e.value.status == wt.WT_ERR_INVALID_ARG
+    with pytest.raises(wt.WtError) as e:
+        wtctx.displace(ok, waves, pos, nrm, wave_count=wt._abi.WT_ERR_INVALID_ARG + 5000)
+    assert e.value.status == wt.WT_ERR_INVALID_ARG
+    hm = torch.zeros((8, 8), device=dev)
+    with pytest.raises(wt.WtError) as e:
+        wtctx.displace(ok, waves, pos, nrm, heightmap=hm, hm_wrap=7)
+    assert e.value.status == wt.WT_ERR_INVALID_ARG
+    # a valid call still works afterwards
+    wtctx.displace(ok, waves, pos, nrm)
+    torch.cuda.synchronize()
+    assert torch.isfinite(pos).all()

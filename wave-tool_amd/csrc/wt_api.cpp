@@ -1,0 +1,484 @@
+// C-ABI of the water-displacement stage: context, validation, dispatch, graphs and
+// the forward autotuner. The reference counterpart of each entry point is named in
+// include/wt.h.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/wt.h"
+#include "wt_host.hpp"
+#include "wt_kernels.hpp"
+
+struct wt_ctx {
+  int device = 0;
+  std::string err;
+  bool forced = false;
+  wt::LaunchShape forced_shape{};
+  std::map<std::pair<int, int>, wt::LaunchShape> table;  // (N bucket, W bucket) -> shape
+};
+
+struct wt_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+wt_status fail(wt_ctx* ctx, wt_status s, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return s;
+}
+
+wt_status hip_fail(wt_ctx* ctx, hipError_t e, const char* where) {
+  return fail(ctx, WT_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+bool is_finite(double v) { return std::isfinite(v); }
+
+int bucket_log2(uint32_t v) {
+  int b = 0;
+  while ((1u << b) < v && b < 31) ++b;
+  return b;
+}
+
+bool valid_shape(const wt::LaunchShape& s) {
+  const bool k_ok = s.steps == 4 || s.steps == 8 || s.steps == 16;
+  const bool lc_ok = s.lanes_per_row == 64 || s.lanes_per_row == 32 || s.lanes_per_row == 16 || s.lanes_per_row == 8;
+  return (s.variant == 1 || s.variant == 2) && k_ok && lc_ok && (s.wave_chunk == 64 || s.wave_chunk == 0);
+}
+
+wt::LaunchShape default_shape(uint32_t N) {
+  // lanes per row no wider than the row, steps so one segment spans about a row
+  wt::LaunchShape s{1, 8, 64, 64};
+  if (N <= 64) s.lanes_per_row = 16;
+  else if (N <= 256) s.lanes_per_row = 32;
+  if (N <= 64) s.steps = 4;
+  return s;
+}
+
+wt::LaunchShape pick_shape(wt_ctx* ctx, uint32_t N, uint32_t W) {
+  if (ctx->forced) return ctx->forced_shape;
+  auto it = ctx->table.find({bucket_log2(N), bucket_log2(W ? W : 1)});
+  if (it != ctx->table.end()) return it->second;
+  return default_shape(N);
+}
+
+bool is_device_pointer(const void* p) {
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged || a.type == hipMemoryTypeUnified;
+}
+
+bool stream_capturing(hipStream_t st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st, &cs) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return cs != hipStreamCaptureStatusNone;
+}
+
+// texture-space scroll offset -s_h * v * t reduced to one wrap period in FP64
+double reduce_offset(double v, int wrap) {
+  const double period = wrap == WT_WRAP_REPEAT ? 1.0 : 2.0;
+  return v - period * std::floor(v / period);
+}
+
+}  // namespace
+
+namespace wt {
+// shared with the batched / adjoint translation units
+wt_status validate_heightmap(wt_ctx* ctx, const wt_heightmap* hm, bool check_ptr) {
+  if (!hm) return WT_OK;
+  if (!hm->texels) return fail(ctx, WT_ERR_INVALID_ARG, "heightmap.texels is NULL");
+  if (hm->width < 1 || hm->height < 1 || hm->width > 16384 || hm->height > 16384)
+    return fail(ctx, WT_ERR_INVALID_ARG, "heightmap size must be in [1, 16384]");
+  if (hm->format != WT_TEXEL_R8_UNORM && hm->format != WT_TEXEL_R32F)
+    return fail(ctx, WT_ERR_INVALID_ARG, "heightmap.format must be R8_UNORM or R32F");
+  if (hm->wrap != WT_WRAP_REPEAT && hm->wrap != WT_WRAP_MIRRORED_REPEAT)
+    return fail(ctx, WT_ERR_INVALID_ARG, "heightmap.wrap must be REPEAT or MIRRORED_REPEAT");
+  if (hm->format == WT_TEXEL_R32F && (reinterpret_cast<uintptr_t>(hm->texels) & 3))
+    return fail(ctx, WT_ERR_INVALID_ARG, "R32F heightmap texels must be 4-byte aligned");
+  if (check_ptr && !is_device_pointer(hm->texels))
+    return fail(ctx, WT_ERR_INVALID_ARG, "heightmap.texels is not device memory");
+  return WT_OK;
+}
+}  // namespace wt
+
+extern "C" {
+
+int wt_abi_version(void) { return WT_ABI_VERSION; }
+
+const char* wt_status_string(wt_status s) {
+  switch (s) {
+    case WT_OK: return "ok";
+    case WT_ERR_INVALID_ARG: return "invalid argument";
+    case WT_ERR_NO_INTERSECTION: return "camera frustum does not intersect the displaceable volume";
+    case WT_ERR_HIP: return "HIP runtime error";
+    case WT_ERR_OOM: return "out of memory";
+    case WT_ERR_UNSUPPORTED: return "unsupported";
+    case WT_ERR_DEGENERATE: return "degenerate projector geometry";
+  }
+  return "unknown status";
+}
+
+wt_status wt_create(wt_ctx** out) {
+  if (!out) return WT_ERR_INVALID_ARG;
+  *out = nullptr;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return WT_ERR_HIP;
+  wt_ctx* c = new (std::nothrow) wt_ctx();
+  if (!c) return WT_ERR_OOM;
+  c->device = dev;
+  *out = c;
+  return WT_OK;
+}
+
+void wt_destroy(wt_ctx* ctx) { delete ctx; }
+
+const char* wt_last_error(const wt_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+static wt::host::Camera make_camera(const wt_camera* c) {
+  wt::host::Camera cam(c->fov_deg, c->aspect, c->near_clip, c->far_clip,
+                       wt::host::Vec3{c->position[0], c->position[1], c->position[2]});
+  cam.set_rotation(c->yaw_deg, c->pitch_deg);
+  return cam;
+}
+
+static bool camera_ok(const wt_camera* c) {
+  if (!c) return false;
+  const float v[9] = {c->fov_deg, c->aspect, c->near_clip, c->far_clip, c->position[0],
+                      c->position[1], c->position[2], c->yaw_deg, c->pitch_deg};
+  for (float x : v)
+    if (!std::isfinite(x)) return false;
+  return c->fov_deg > 0.0f && c->fov_deg < 180.0f && c->aspect > 0.0f && c->near_clip > 0.0f &&
+         c->far_clip > c->near_clip;
+}
+
+wt_status wt_camera_matrices(const wt_camera* cam, double view[16], double proj[16], double forward[3]) {
+  if (!camera_ok(cam) || !view || !proj || !forward) return WT_ERR_INVALID_ARG;
+  wt::host::Camera c = make_camera(cam);
+  for (int i = 0; i < 16; ++i) {
+    view[i] = c.view.c[i / 4][i % 4];
+    proj[i] = c.proj.c[i / 4][i % 4];
+  }
+  forward[0] = c.forward.x;
+  forward[1] = c.forward.y;
+  forward[2] = c.forward.z;
+  return WT_OK;
+}
+
+wt_status wt_projector_corners(const wt_camera* cam, float amp, double corners[4][4]) {
+  if (!camera_ok(cam) || !corners || !std::isfinite(amp) || amp < 0.0f) return WT_ERR_INVALID_ARG;
+  wt::host::Camera c = make_camera(cam);
+  const int st = wt::host::projector_corners(c, (double)amp, corners);
+  if (st == wt::host::PROJ_NO_INTERSECTION) return WT_ERR_NO_INTERSECTION;
+  if (st != wt::host::PROJ_OK) return WT_ERR_DEGENERATE;
+  return WT_OK;
+}
+
+wt_status wt_normalize_steepness(wt_gerstner_wave* waves, const float* q, uint32_t count, uint32_t norm_count) {
+  if ((!waves || !q) && count) return WT_ERR_INVALID_ARG;
+  if (norm_count == 0 && count) return WT_ERR_INVALID_ARG;
+  for (uint32_t i = 0; i < count; ++i) {
+    // render-engine.cpp:1073, in FP32 like the reference
+    const float wa = waves[i].frequency_w * waves[i].amplitude_A;
+    waves[i].steepness_Q_i = wa != 0.0f ? q[i] / (wa * (float)norm_count) : 0.0f;
+  }
+  return WT_OK;
+}
+
+wt_status wt_water_displace(wt_ctx* ctx, const wt_water_params* p, const wt_gerstner_wave* d_waves,
+                            uint32_t W, const wt_heightmap* hm, const wt_vertex_buffers* out, void* stream) {
+  if (!ctx) return WT_ERR_INVALID_ARG;
+  if (!p || !out) return fail(ctx, WT_ERR_INVALID_ARG, "params and out must be non-NULL");
+  const uint32_t N = p->gridLength;
+  if (N < 2 || N > 32768) return fail(ctx, WT_ERR_INVALID_ARG, "gridLength must be in [2, 32768]");
+  if (W > WT_MAX_WAVES) return fail(ctx, WT_ERR_INVALID_ARG, "wave_count exceeds WT_MAX_WAVES");
+  if (W && !d_waves) return fail(ctx, WT_ERR_INVALID_ARG, "d_waves is NULL");
+  if (!out->position || !out->normal) return fail(ctx, WT_ERR_INVALID_ARG, "position and normal buffers are required");
+  const void* bufs[4] = {out->position, out->normal, out->tangent, out->bitangent};
+  for (const void* b : bufs)
+    if (b && (reinterpret_cast<uintptr_t>(b) & 3)) return fail(ctx, WT_ERR_INVALID_ARG, "output buffers must be 4-byte aligned");
+  for (int i = 0; i < 4; ++i)
+    for (int k = 0; k < 4; ++k)
+      if (!is_finite(p->corners[i][k])) return fail(ctx, WT_ERR_INVALID_ARG, "corners must be finite");
+  const float fs[8] = {p->cameraPosition[0], p->cameraPosition[1], p->cameraPosition[2], p->heightmapSampleScale,
+                       p->heightmapDisplacementScale, p->verticalBounceWaveDisplacement,
+                       p->heightmapScrollVelocity[0], p->heightmapScrollVelocity[1]};
+  for (float v : fs)
+    if (!std::isfinite(v)) return fail(ctx, WT_ERR_INVALID_ARG, "scalar parameters must be finite");
+  if (!is_finite(p->waveAnimationTimeInSeconds)) return fail(ctx, WT_ERR_INVALID_ARG, "time must be finite");
+  hipStream_t st = (hipStream_t)stream;
+  const bool capturing = stream_capturing(st);
+  wt_status hs = wt::validate_heightmap(ctx, hm, !capturing);
+  if (hs != WT_OK) return hs;
+  if (!capturing) {
+    for (const void* b : bufs)
+      if (b && !is_device_pointer(b)) return fail(ctx, WT_ERR_INVALID_ARG, "output buffers must be device memory");
+    if (W && !is_device_pointer(d_waves)) return fail(ctx, WT_ERR_INVALID_ARG, "d_waves must be device memory");
+  }
+
+  wt::FwdParams fp{};
+  const double(*c)[4] = p->corners;  // [0] BL, [1] TL, [2] BR, [3] TR
+  fp.bl_x = c[0][0];
+  fp.bl_z = c[0][2];
+  fp.du_x = c[2][0] - c[0][0];
+  fp.du_z = c[2][2] - c[0][2];
+  fp.dv_x = c[1][0] - c[0][0];
+  fp.dv_z = c[1][2] - c[0][2];
+  fp.duv_x = c[3][0] - c[1][0] - c[2][0] + c[0][0];
+  fp.duv_z = c[3][2] - c[1][2] - c[2][2] + c[0][2];
+  fp.inv_nm1 = 1.0 / (double)(N - 1);
+  fp.time = p->waveAnimationTimeInSeconds;
+  fp.N = (int)N;
+  fp.W = (int)W;
+  fp.waves = d_waves;
+  wt::EpilogueArgs& e = fp.ep;
+  e.cam[0] = p->cameraPosition[0];
+  e.cam[1] = p->cameraPosition[1];
+  e.cam[2] = p->cameraPosition[2];
+  e.s_h = p->heightmapSampleScale;
+  e.s_d = p->heightmapDisplacementScale;
+  e.bounce = p->verticalBounceWaveDisplacement;
+  e.pos = out->position;
+  e.nrm = out->normal;
+  e.tan = out->tangent;
+  e.bit = out->bitangent;
+  if (hm) {
+    e.texels = hm->texels;
+    e.tex_w = (int)hm->width;
+    e.tex_h = (int)hm->height;
+    e.tex_fmt = (int)hm->format;
+    e.tex_wrap = (int)hm->wrap;
+    const double t = p->waveAnimationTimeInSeconds, sh = p->heightmapSampleScale;
+    e.hm_off_s = reduce_offset(-sh * (double)p->heightmapScrollVelocity[0] * t, e.tex_wrap);
+    e.hm_off_t = reduce_offset(sh * (double)p->heightmapScrollVelocity[1] * t, e.tex_wrap);
+  }
+  const wt::LaunchShape shape = pick_shape(ctx, N, W);
+  hipError_t he = wt::launch_forward(fp, shape, st);
+  if (he != hipSuccess) return hip_fail(ctx, he, "water_forward_kernel launch");
+  return WT_OK;
+}
+
+wt_status wt_set_launch_config(wt_ctx* ctx, const wt_launch_config* cfg) {
+  if (!ctx || !cfg) return WT_ERR_INVALID_ARG;
+  if (!cfg->variant && !cfg->steps_per_lane && !cfg->lanes_per_row && !cfg->wave_chunk) {
+    ctx->forced = false;
+    return WT_OK;
+  }
+  wt::LaunchShape s{(int)cfg->variant, (int)cfg->steps_per_lane, (int)cfg->lanes_per_row,
+                    cfg->wave_chunk ? (int)cfg->wave_chunk : 64};
+  if (!valid_shape(s)) return fail(ctx, WT_ERR_INVALID_ARG, "unsupported launch configuration");
+  ctx->forced = true;
+  ctx->forced_shape = s;
+  return WT_OK;
+}
+
+wt_status wt_get_launch_config(wt_ctx* ctx, uint32_t N, uint32_t W, wt_launch_config* cfg) {
+  if (!ctx || !cfg) return WT_ERR_INVALID_ARG;
+  const wt::LaunchShape s = pick_shape(ctx, N, W);
+  cfg->variant = (uint32_t)s.variant;
+  cfg->steps_per_lane = (uint32_t)s.steps;
+  cfg->lanes_per_row = (uint32_t)s.lanes_per_row;
+  cfg->wave_chunk = (uint32_t)(s.wave_chunk ? s.wave_chunk : 64);
+  return WT_OK;
+}
+
+wt_status wt_autotune_forward(wt_ctx* ctx, uint32_t N, uint32_t W, int with_hm, wt_launch_config* best,
+                              float* best_ms) {
+  if (!ctx) return WT_ERR_INVALID_ARG;
+  if (N < 2 || N > 8192 || W > WT_MAX_WAVES) return fail(ctx, WT_ERR_INVALID_ARG, "autotune size out of range");
+  const size_t V = (size_t)N * N;
+  float *pos = nullptr, *nrm = nullptr, *tex = nullptr;
+  wt_gerstner_wave* dw = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  wt_status rs = WT_OK;
+  const uint32_t TW = 256;
+  std::vector<wt_gerstner_wave> hw(W ? W : 1);
+  for (uint32_t i = 0; i < W; ++i) {  // deterministic synthetic wave set
+    const float L = 1.0f + 59.0f * (float)((i * 37u) % 101u) / 100.0f;
+    const float ang = 0.7f * std::sin(1.7f * (float)i);
+    hw[i] = wt_gerstner_wave{L / 90.0f, 6.2831853f / L, std::sqrt(9.81f * 6.2831853f / L), 0.5f / (float)W,
+                             {std::cos(ang), std::sin(ang)}, 0.3f * (float)i, 0.0f};
+  }
+  std::vector<float> htex(TW * TW);
+  for (uint32_t i = 0; i < TW * TW; ++i) htex[i] = 0.5f + 0.5f * std::sin(0.01f * (float)i);
+  auto cleanup = [&]() {
+    if (pos) (void)hipFree(pos);
+    if (nrm) (void)hipFree(nrm);
+    if (tex) (void)hipFree(tex);
+    if (dw) (void)hipFree(dw);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+  };
+#define WT_TRY(x)                                \
+  do {                                           \
+    hipError_t _e = (x);                         \
+    if (_e != hipSuccess) {                      \
+      rs = hip_fail(ctx, _e, #x);                \
+      cleanup();                                 \
+      return rs;                                 \
+    }                                            \
+  } while (0)
+  WT_TRY(hipMalloc(&pos, V * 12));
+  WT_TRY(hipMalloc(&nrm, V * 12));
+  WT_TRY(hipMalloc(&dw, sizeof(wt_gerstner_wave) * hw.size()));
+  WT_TRY(hipMalloc(&tex, sizeof(float) * htex.size()));
+  WT_TRY(hipMemcpy(dw, hw.data(), sizeof(wt_gerstner_wave) * hw.size(), hipMemcpyHostToDevice));
+  WT_TRY(hipMemcpy(tex, htex.data(), sizeof(float) * htex.size(), hipMemcpyHostToDevice));
+  WT_TRY(hipStreamCreate(&st));
+  WT_TRY(hipEventCreate(&e0));
+  WT_TRY(hipEventCreate(&e1));
+  wt_water_params p{};
+  const double cc[4][4] = {{-60, 0, -10, 1}, {-900, 0, -1000, 1}, {60, 0, -10, 1}, {900, 0, -1000, 1}};
+  std::memcpy(p.corners, cc, sizeof(cc));
+  p.cameraPosition[1] = 10.0f;
+  p.gridLength = N;
+  p.heightmapSampleScale = 0.02f;
+  p.heightmapDisplacementScale = 0.05f;
+  p.waveAnimationTimeInSeconds = 10.0;
+  wt_heightmap hm{tex, TW, TW, WT_TEXEL_R32F, WT_WRAP_REPEAT};
+  wt_vertex_buffers vb{pos, nrm, nullptr, nullptr};
+  const bool saved_forced = ctx->forced;
+  const wt::LaunchShape saved_shape = ctx->forced_shape;
+  float best_t = 1e30f;
+  wt::LaunchShape best_s = default_shape(N);
+  const int vars[2] = {1, 2}, ks[3] = {4, 8, 16}, lcs[4] = {64, 32, 16, 8};
+  for (int var : vars)
+    for (int k : ks)
+      for (int lc : lcs) {
+        if ((uint32_t)lc > N && lc != 8) continue;
+        ctx->forced = true;
+        ctx->forced_shape = wt::LaunchShape{var, k, lc, 64};
+        p.waveAnimationTimeInSeconds = 10.0;
+        wt_status s0 = wt_water_displace(ctx, &p, dw, W, with_hm ? &hm : nullptr, &vb, st);
+        if (s0 != WT_OK) {
+          ctx->forced = saved_forced;
+          ctx->forced_shape = saved_shape;
+          cleanup();
+          return s0;
+        }
+        const int reps = 5;
+        WT_TRY(hipEventRecord(e0, st));
+        for (int r = 0; r < reps; ++r) {
+          p.waveAnimationTimeInSeconds = 10.0 + r / 60.0;
+          (void)wt_water_displace(ctx, &p, dw, W, with_hm ? &hm : nullptr, &vb, st);
+        }
+        WT_TRY(hipEventRecord(e1, st));
+        WT_TRY(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        WT_TRY(hipEventElapsedTime(&ms, e0, e1));
+        ms /= reps;
+        if (ms < best_t) {
+          best_t = ms;
+          best_s = ctx->forced_shape;
+        }
+      }
+#undef WT_TRY
+  ctx->forced = saved_forced;
+  ctx->forced_shape = saved_shape;
+  ctx->table[{bucket_log2(N), bucket_log2(W ? W : 1)}] = best_s;
+  if (best) {
+    best->variant = (uint32_t)best_s.variant;
+    best->steps_per_lane = (uint32_t)best_s.steps;
+    best->lanes_per_row = (uint32_t)best_s.lanes_per_row;
+    best->wave_chunk = 64;
+  }
+  if (best_ms) *best_ms = best_t;
+  cleanup();
+  return WT_OK;
+}
+
+wt_status wt_autotune_save(wt_ctx* ctx, const char* path) {
+  if (!ctx || !path) return WT_ERR_INVALID_ARG;
+  FILE* f = std::fopen(path, "w");
+  if (!f) return fail(ctx, WT_ERR_INVALID_ARG, std::string("cannot open ") + path);
+  std::fprintf(f, "# wt forward autotune: n_bucket w_bucket variant steps lanes_per_row wave_chunk\n");
+  for (const auto& kv : ctx->table)
+    std::fprintf(f, "%d %d %d %d %d %d\n", kv.first.first, kv.first.second, kv.second.variant, kv.second.steps,
+                 kv.second.lanes_per_row, kv.second.wave_chunk);
+  std::fclose(f);
+  return WT_OK;
+}
+
+wt_status wt_autotune_load(wt_ctx* ctx, const char* path) {
+  if (!ctx || !path) return WT_ERR_INVALID_ARG;
+  FILE* f = std::fopen(path, "r");
+  if (!f) return fail(ctx, WT_ERR_INVALID_ARG, std::string("cannot open ") + path);
+  char line[256];
+  std::map<std::pair<int, int>, wt::LaunchShape> t;
+  while (std::fgets(line, sizeof line, f)) {
+    if (line[0] == '#') continue;
+    int nb, wb;
+    wt::LaunchShape s{};
+    if (std::sscanf(line, "%d %d %d %d %d %d", &nb, &wb, &s.variant, &s.steps, &s.lanes_per_row, &s.wave_chunk) != 6 ||
+        !valid_shape(s)) {
+      std::fclose(f);
+      return fail(ctx, WT_ERR_INVALID_ARG, "malformed autotune table");
+    }
+    t[{nb, wb}] = s;
+  }
+  std::fclose(f);
+  ctx->table = t;
+  return WT_OK;
+}
+
+wt_status wt_graph_begin(wt_ctx* ctx, void* stream) {
+  if (!ctx) return WT_ERR_INVALID_ARG;
+  if (!stream) return fail(ctx, WT_ERR_INVALID_ARG, "graph capture needs a non-default stream");
+  hipError_t e = hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamBeginCapture");
+  return WT_OK;
+}
+
+wt_status wt_graph_end(wt_ctx* ctx, void* stream, wt_graph** out) {
+  if (!ctx || !out) return WT_ERR_INVALID_ARG;
+  *out = nullptr;
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture((hipStream_t)stream, &g);
+  if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamEndCapture");
+  hipGraphExec_t x = nullptr;
+  e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return hip_fail(ctx, e, "hipGraphInstantiate");
+  }
+  wt_graph* r = new (std::nothrow) wt_graph();
+  if (!r) {
+    (void)hipGraphExecDestroy(x);
+    (void)hipGraphDestroy(g);
+    return fail(ctx, WT_ERR_OOM, "graph handle");
+  }
+  r->graph = g;
+  r->exec = x;
+  *out = r;
+  return WT_OK;
+}
+
+wt_status wt_graph_launch(wt_graph* g, void* stream) {
+  if (!g) return WT_ERR_INVALID_ARG;
+  return hipGraphLaunch(g->exec, (hipStream_t)stream) == hipSuccess ? WT_OK : WT_ERR_HIP;
+}
+
+void wt_graph_destroy(wt_graph* g) {
+  if (!g) return;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+}
+
+}  // extern "C"

@@ -1,0 +1,173 @@
+// Device-side building blocks shared by the forward, batched and adjoint kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/wt.h"
+
+namespace wt {
+
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+constexpr double kInvTwoPi = 0.15915494309189533576888376337251;
+
+// Per-wave constants staged in LDS, shared by every wavefront of a workgroup.
+// Phases are carried in revolutions (1 rev = 2 pi rad), the unit of v_sin_f32 /
+// v_cos_f32. kx, kz: revolutions per metre along world x and z (w Dx / 2pi,
+// w Dz / 2pi); c0: fract((phi t + phi0) / 2pi) reduced in FP64, so the FP32 part of
+// the kernel never sees a large phase.
+struct WaveLds {
+  double kx, kz, c0;
+  float cc[4];  // multiply cos: Q A Dx (x disp), Q A Dz (z disp), A w Dx (dy/dx), A w Dz (dy/dz)
+  float cs[4];  // multiply sin: A (y disp), Q A w Dx^2, Q A w Dx Dz, Q A w Dz^2
+};
+
+// fract of an FP64 phase in revolutions (v_fract_f64), rounded to FP32: the
+// argument of v_sin_f32 / v_cos_f32 stays in [0, 1] revolution.
+__device__ __forceinline__ float reduce_rev(double r) { return (float)__builtin_amdgcn_fract(r); }
+
+__device__ __forceinline__ void stage_wave(const wt_gerstner_wave& g, double t, WaveLds* o) {
+  const double w = (double)g.frequency_w, dx = (double)g.xzDirection_D[0], dz = (double)g.xzDirection_D[1];
+  o->kx = w * dx * kInvTwoPi;
+  o->kz = w * dz * kInvTwoPi;
+  const double ph = ((double)g.phaseConstant_phi * t + (double)g.phaseOffset_phi0) * kInvTwoPi;
+  o->c0 = ph - floor(ph);
+  const float A = g.amplitude_A, Q = g.steepness_Q_i, wf = g.frequency_w;
+  const float Dx = g.xzDirection_D[0], Dz = g.xzDirection_D[1];
+  const float qa = Q * A, aw = A * wf, qaw = qa * wf;
+  o->cc[0] = qa * Dx;
+  o->cc[1] = qa * Dz;
+  o->cc[2] = aw * Dx;
+  o->cc[3] = aw * Dz;
+  o->cs[0] = A;
+  o->cs[1] = qaw * Dx * Dx;
+  o->cs[2] = qaw * Dx * Dz;
+  o->cs[3] = qaw * Dz * Dz;
+}
+
+// Bilinear heightmap tap with GL texel-centre and wrap rules. (s, t) are texture
+// coordinates; returns the texel-space value and dH/ds, dH/dt of the same cell.
+// Index arithmetic is clamped so any finite or non-finite input stays in bounds.
+__device__ __forceinline__ float heightmap_tap(const void* __restrict__ texels, int W, int H, int fmt, int wrap_mode,
+                                               float s, float t, float* dds, float* ddt) {
+  const bool repeat = wrap_mode == WT_WRAP_REPEAT;
+  // reduce to one period first so the texel coordinate stays small
+  if (repeat) {
+    s = s - floorf(s);
+    t = t - floorf(t);
+  } else {
+    s = s - 2.0f * floorf(0.5f * s);
+    t = t - 2.0f * floorf(0.5f * t);
+  }
+  const int period_w = repeat ? W : 2 * W;
+  const int period_h = repeat ? H : 2 * H;
+  const float u = s * (float)W - 0.5f, v = t * (float)H - 0.5f;
+  const float fu = floorf(u), fv = floorf(v);
+  const float a = u - fu, b = v - fv;
+  // fmaxf/fminf return the non-NaN operand, so a NaN coordinate lands on texel -1
+  int i0 = (int)fminf(fmaxf(fu, -1.0f), (float)(period_w - 1));
+  int j0 = (int)fminf(fmaxf(fv, -1.0f), (float)(period_h - 1));
+  int i1 = i0 + 1, j1 = j0 + 1;
+  auto wrap = [repeat](int i, int n, int period) {
+    i = i < 0 ? i + period : (i >= period ? i - period : i);
+    if (!repeat) i = i < n ? i : (2 * n - 1 - i);
+    return i;
+  };
+  i0 = wrap(i0, W, period_w);
+  i1 = wrap(i1, W, period_w);
+  j0 = wrap(j0, H, period_h);
+  j1 = wrap(j1, H, period_h);
+  float t00, t10, t01, t11;
+  if (fmt == WT_TEXEL_R8_UNORM) {
+    const uint8_t* p = (const uint8_t*)texels;
+    const float k = 1.0f / 255.0f;
+    t00 = (float)p[j0 * W + i0] * k;
+    t10 = (float)p[j0 * W + i1] * k;
+    t01 = (float)p[j1 * W + i0] * k;
+    t11 = (float)p[j1 * W + i1] * k;
+  } else {
+    const float* p = (const float*)texels;
+    t00 = p[j0 * W + i0];
+    t10 = p[j0 * W + i1];
+    t01 = p[j1 * W + i0];
+    t11 = p[j1 * W + i1];
+  }
+  const float lo = fmaf(a, t10 - t00, t00), hi = fmaf(a, t11 - t01, t01);
+  *dds = (float)W * fmaf(b, (t11 - t01) - (t10 - t00), t10 - t00);
+  *ddt = (float)H * fmaf(a, (t11 - t10) - (t01 - t00), t01 - t00);
+  return fmaf(b, hi - lo, lo);
+}
+
+// Per-vertex epilogue: assemble position and partials, add the heightmap bump at
+// the displaced point and the bounce, form N / T / B, store.
+struct EpilogueArgs {
+  float cam[3];
+  float s_h, s_d, bounce;
+  double hm_off_s, hm_off_t;  // texture-space scroll offsets, already reduced to one period
+  const void* texels;
+  int tex_w, tex_h, tex_fmt, tex_wrap;
+  float* pos;
+  float* nrm;
+  float* tan;
+  float* bit;
+};
+
+// acc: X, Z, Cx, Cz (cos sums) and Y, Sxx, Sxz, Szz (sin sums).
+template <int kHm>
+__device__ __forceinline__ void vertex_epilogue(const EpilogueArgs& e, double x, double z, const float acc[8],
+                                                size_t vid) {
+  const float X = acc[0], Z = acc[1], Cx = acc[2], Cz = acc[3];
+  const float Y = acc[4], Sxx = acc[5], Sxz = acc[6], Szz = acc[7];
+  const double pxd = x + (double)X, pzd = z + (double)Z;
+  const float px = (float)pxd, pz = (float)pzd;
+  float py = Y;
+  // dP/dx and dP/dz of the Gerstner surface (no reference counterpart: the
+  // reference differences neighbours, water-grid.vert:164-187)
+  float tx = 1.0f - Sxx, ty = Cx, tz = -Sxz;
+  float bx = -Sxz, by = Cz, bz = 1.0f - Szz;
+  if (kHm) {
+    // water-grid.vert:103-110: sample at s_h * (x, -z) of the displaced point. The
+    // texture coordinate is formed from the FP64 displaced point and reduced to one
+    // wrap period in FP64, so its FP32 rounding is relative to a number below 2.
+    float dds, ddt;
+    double sd = fma((double)e.s_h, pxd, e.hm_off_s), td = fma(-(double)e.s_h, pzd, e.hm_off_t);
+    if (e.tex_wrap == WT_WRAP_REPEAT) {
+      sd -= floor(sd);
+      td -= floor(td);
+    } else {
+      sd -= 2.0 * floor(0.5 * sd);
+      td -= 2.0 * floor(0.5 * td);
+    }
+    const float s = (float)sd, t = (float)td;
+    const float r = heightmap_tap(e.texels, e.tex_w, e.tex_h, e.tex_fmt, e.tex_wrap, s, t, &dds, &ddt);
+    py += e.s_d * (2.0f * (r - 0.5f));
+    const float k = 2.0f * e.s_d * e.s_h;
+    const float hpx = k * dds, hpz = -k * ddt;  // d(bump)/dPx, d(bump)/dPz
+    ty += hpx * tx + hpz * tz;
+    by += hpx * bx + hpz * bz;
+  }
+  py += e.bounce;  // water-grid.vert:147
+  // N = normalize(B x T): y-up for the undisplaced plane
+  float nx = by * tz - bz * ty, ny = bz * tx - bx * tz, nz = bx * ty - by * tx;
+  const float inv_n = 1.0f / sqrtf(nx * nx + ny * ny + nz * nz);
+  nx *= inv_n; ny *= inv_n; nz *= inv_n;
+  // flip toward the camera (water-grid.vert:190)
+  const float vx = e.cam[0] - px, vy = e.cam[1] - py, vz = e.cam[2] - pz;
+  if (nx * vx + ny * vy + nz * vz < 0.0f) { nx = -nx; ny = -ny; nz = -nz; }
+  float* P = e.pos + 3 * vid;
+  P[0] = px; P[1] = py; P[2] = pz;
+  float* Nn = e.nrm + 3 * vid;
+  Nn[0] = nx; Nn[1] = ny; Nn[2] = nz;
+  if (e.tan) {
+    const float it = 1.0f / sqrtf(tx * tx + ty * ty + tz * tz);
+    float* Tt = e.tan + 3 * vid;
+    Tt[0] = tx * it; Tt[1] = ty * it; Tt[2] = tz * it;
+  }
+  if (e.bit) {
+    const float ib = 1.0f / sqrtf(bx * bx + by * by + bz * bz);
+    float* Bb = e.bit + 3 * vid;
+    Bb[0] = bx * ib; Bb[1] = by * ib; Bb[2] = bz * ib;
+  }
+}
+
+}  // namespace wt

@@ -1,0 +1,164 @@
+"""Python side of the MI355X water-displacement stage (ctypes over libwavetool.so).
+
+The product path is the C ABI in include/wt.h; this module only marshals
+arguments (device memory and streams come from torch tensors) and raises on
+non-OK status. There is no CPU fallback: every compute call goes to the HIP
+kernels in libwavetool.so.
+"""
+import ctypes
+
+from . import scenes
+from ._abi import (LaunchConfig, Camera, GerstnerWave, Heightmap, VertexBuffers, WaterParams, WtError, check, lib,
+                   WT_OK, WT_ERR_INVALID_ARG, WT_ERR_NO_INTERSECTION, WT_ERR_DEGENERATE, WT_WRAP_REPEAT,
+                   WT_WRAP_MIRRORED_REPEAT, WT_TEXEL_R8_UNORM, WT_TEXEL_R32F, WT_FLOP_PER_VERTEX_WAVE,
+                   WT_BYTES_PER_VERTEX_PN, WT_BYTES_PER_VERTEX_PNTB)
+
+__all__ = ["Context", "camera", "projector_corners", "water_params", "normalize_steepness", "scenes", "lib", "check",
+           "WtError"]
+
+
+def camera(cam_tuple):
+    """Camera from (fov_deg, aspect, near, far, px, py, pz, yaw_deg, pitch_deg)."""
+    c = Camera()
+    c.fov_deg, c.aspect, c.near_clip, c.far_clip = cam_tuple[0:4]
+    c.position[:] = list(cam_tuple[4:7])
+    c.yaw_deg, c.pitch_deg = cam_tuple[7], cam_tuple[8]
+    return c
+
+
+def projector_corners(cam_tuple, amp):
+    """World grid corners [4][4] (BL, TL, BR, TR) from the host projector (double)."""
+    import numpy as np
+    out = (ctypes.c_double * 16)()
+    st = lib().wt_projector_corners(ctypes.byref(camera(cam_tuple)), float(amp), out)
+    check(st)
+    return np.array(out[:], dtype=np.float64).reshape(4, 4)
+
+
+def camera_matrices(cam_tuple):
+    import numpy as np
+    view, proj, fwd = (ctypes.c_double * 16)(), (ctypes.c_double * 16)(), (ctypes.c_double * 3)()
+    check(lib().wt_camera_matrices(ctypes.byref(camera(cam_tuple)), view, proj, fwd))
+    return (np.array(view[:]).reshape(4, 4).T.copy(), np.array(proj[:]).reshape(4, 4).T.copy(),
+            np.array(fwd[:]))
+
+
+def normalize_steepness(waves, q, normaliser):
+    """In-place Q_i = q / (w A normaliser) on a (W, 8) float32 wave array."""
+    import numpy as np
+    W = waves.shape[0]
+    arr = (GerstnerWave * W)()
+    ctypes.memmove(arr, waves.ctypes.data, W * 32)
+    qa = np.ascontiguousarray(q, dtype=np.float32)
+    check(lib().wt_normalize_steepness(arr, qa.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), W, normaliser))
+    ctypes.memmove(waves.ctypes.data, arr, W * 32)
+    return waves
+
+
+def water_params(corners, cam_pos, N, time, sample_scale=0.0, displacement_scale=0.0, bounce=0.0,
+                 scroll=(0.0, 0.0)):
+    p = WaterParams()
+    for i in range(4):
+        for k in range(4):
+            p.corners[i][k] = float(corners[i][k])
+    p.cameraPosition[:] = [float(v) for v in cam_pos]
+    p.gridLength = int(N)
+    p.heightmapSampleScale = float(sample_scale)
+    p.heightmapDisplacementScale = float(displacement_scale)
+    p.verticalBounceWaveDisplacement = float(bounce)
+    p.heightmapScrollVelocity[:] = [float(scroll[0]), float(scroll[1])]
+    p.waveAnimationTimeInSeconds = float(time)
+    return p
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+class Context:
+    """Owns a wt_ctx bound to the current device."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        check(lib().wt_create(ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().wt_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        check(st, self.handle)
+
+    def displace(self, params, waves, pos, nrm, tan=None, bit=None, heightmap=None, hm_format=WT_TEXEL_R32F,
+                 hm_wrap=WT_WRAP_REPEAT, stream=None, wave_count=None):
+        """waves: device tensor (W, 8) float32; pos/nrm/tan/bit: device float tensors
+        with gridLength^2 * 3 elements; heightmap: device tensor (H, W) or None."""
+        W = int(waves.shape[0]) if wave_count is None else int(wave_count)
+        vb = VertexBuffers(_ptr(pos), _ptr(nrm), _ptr(tan), _ptr(bit))
+        hm = None
+        if heightmap is not None:
+            h, w = heightmap.shape
+            hm = Heightmap(heightmap.data_ptr(), w, h, hm_format, hm_wrap)
+        st = lib().wt_water_displace(self.handle, ctypes.byref(params), _ptr(waves) if W else None, W,
+                                     ctypes.byref(hm) if hm is not None else None, ctypes.byref(vb), _stream(stream))
+        self._check(st)
+
+    def set_launch_config(self, variant=0, steps=0, lanes_per_row=0, wave_chunk=0):
+        cfg = LaunchConfig(variant, steps, lanes_per_row, wave_chunk)
+        self._check(lib().wt_set_launch_config(self.handle, ctypes.byref(cfg)))
+
+    def get_launch_config(self, N, W):
+        cfg = LaunchConfig()
+        self._check(lib().wt_get_launch_config(self.handle, N, W, ctypes.byref(cfg)))
+        return dict(variant=cfg.variant, steps=cfg.steps_per_lane, lanes_per_row=cfg.lanes_per_row,
+                    wave_chunk=cfg.wave_chunk)
+
+    def autotune(self, N, W, with_heightmap=True):
+        cfg = LaunchConfig()
+        ms = ctypes.c_float()
+        self._check(lib().wt_autotune_forward(self.handle, N, W, int(bool(with_heightmap)), ctypes.byref(cfg),
+                                              ctypes.byref(ms)))
+        return dict(variant=cfg.variant, steps=cfg.steps_per_lane, lanes_per_row=cfg.lanes_per_row,
+                    wave_chunk=cfg.wave_chunk), ms.value
+
+    def graph_begin(self, stream):
+        self._check(lib().wt_graph_begin(self.handle, ctypes.c_void_p(stream.cuda_stream)))
+
+    def graph_end(self, stream):
+        g = ctypes.c_void_p()
+        self._check(lib().wt_graph_end(self.handle, ctypes.c_void_p(stream.cuda_stream), ctypes.byref(g)))
+        return Graph(g)
+
+
+class Graph:
+    def __init__(self, handle):
+        self.handle = handle
+
+    def launch(self, stream):
+        check(lib().wt_graph_launch(self.handle, ctypes.c_void_p(stream.cuda_stream)))
+
+    def close(self):
+        if self.handle:
+            lib().wt_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,164 @@
+"""ctypes binding of include/wt.h (libwavetool.so, built in-tree for gfx950).
+
+The shared library links the HIP runtime by soname. torch is imported first so
+that the library binds to the same runtime instance torch already loaded, and
+device pointers from torch tensors are valid in both.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libwavetool.so")
+
+WT_OK = 0
+WT_ERR_INVALID_ARG = 1
+WT_ERR_NO_INTERSECTION = 2
+WT_ERR_HIP = 3
+WT_ERR_OOM = 4
+WT_ERR_UNSUPPORTED = 5
+WT_ERR_DEGENERATE = 6
+
+WT_WRAP_REPEAT = 0
+WT_WRAP_MIRRORED_REPEAT = 1
+WT_TEXEL_R8_UNORM = 0
+WT_TEXEL_R32F = 1
+
+WT_FLOP_PER_VERTEX_WAVE = 26
+WT_BYTES_PER_VERTEX_PN = 24
+WT_BYTES_PER_VERTEX_PNTB = 48
+WT_ADJOINT_FLOP_PER_VERTEX_WAVE_SNAPSHOT = 50
+
+
+class GerstnerWave(ctypes.Structure):
+    _fields_ = [
+        ("amplitude_A", ctypes.c_float),
+        ("frequency_w", ctypes.c_float),
+        ("phaseConstant_phi", ctypes.c_float),
+        ("steepness_Q_i", ctypes.c_float),
+        ("xzDirection_D", ctypes.c_float * 2),
+        ("phaseOffset_phi0", ctypes.c_float),
+        ("reserved", ctypes.c_float),
+    ]
+
+
+class Camera(ctypes.Structure):
+    _fields_ = [
+        ("fov_deg", ctypes.c_float),
+        ("aspect", ctypes.c_float),
+        ("near_clip", ctypes.c_float),
+        ("far_clip", ctypes.c_float),
+        ("position", ctypes.c_float * 3),
+        ("yaw_deg", ctypes.c_float),
+        ("pitch_deg", ctypes.c_float),
+    ]
+
+
+class Heightmap(ctypes.Structure):
+    _fields_ = [
+        ("texels", ctypes.c_void_p),
+        ("width", ctypes.c_uint32),
+        ("height", ctypes.c_uint32),
+        ("format", ctypes.c_uint32),
+        ("wrap", ctypes.c_uint32),
+    ]
+
+
+class WaterParams(ctypes.Structure):
+    _fields_ = [
+        ("corners", (ctypes.c_double * 4) * 4),
+        ("cameraPosition", ctypes.c_float * 3),
+        ("gridLength", ctypes.c_uint32),
+        ("heightmapSampleScale", ctypes.c_float),
+        ("heightmapDisplacementScale", ctypes.c_float),
+        ("verticalBounceWaveDisplacement", ctypes.c_float),
+        ("heightmapScrollVelocity", ctypes.c_float * 2),
+        ("reserved", ctypes.c_float),
+        ("waveAnimationTimeInSeconds", ctypes.c_double),
+    ]
+
+
+class VertexBuffers(ctypes.Structure):
+    _fields_ = [
+        ("position", ctypes.c_void_p),
+        ("normal", ctypes.c_void_p),
+        ("tangent", ctypes.c_void_p),
+        ("bitangent", ctypes.c_void_p),
+    ]
+
+
+class LaunchConfig(ctypes.Structure):
+    _fields_ = [
+        ("variant", ctypes.c_uint32),
+        ("steps_per_lane", ctypes.c_uint32),
+        ("lanes_per_row", ctypes.c_uint32),
+        ("wave_chunk", ctypes.c_uint32),
+    ]
+
+
+# (name, restype, argtypes) for every entry point declared in include/wt.h
+_vp = ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+PROTOTYPES = [
+    ("wt_abi_version", ctypes.c_int, []),
+    ("wt_status_string", ctypes.c_char_p, [ctypes.c_int]),
+    ("wt_create", ctypes.c_int, [ctypes.POINTER(_vp)]),
+    ("wt_destroy", None, [_vp]),
+    ("wt_last_error", ctypes.c_char_p, [_vp]),
+    ("wt_camera_matrices", ctypes.c_int, [ctypes.POINTER(Camera), _dp, _dp, _dp]),
+    ("wt_projector_corners", ctypes.c_int, [ctypes.POINTER(Camera), ctypes.c_float, _dp]),
+    ("wt_normalize_steepness", ctypes.c_int,
+     [ctypes.POINTER(GerstnerWave), ctypes.POINTER(ctypes.c_float), ctypes.c_uint32, ctypes.c_uint32]),
+    ("wt_water_displace", ctypes.c_int,
+     [_vp, ctypes.POINTER(WaterParams), _vp, ctypes.c_uint32, ctypes.POINTER(Heightmap),
+      ctypes.POINTER(VertexBuffers), _vp]),
+    ("wt_set_launch_config", ctypes.c_int, [_vp, ctypes.POINTER(LaunchConfig)]),
+    ("wt_get_launch_config", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(LaunchConfig)]),
+    ("wt_autotune_forward", ctypes.c_int,
+     [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(LaunchConfig),
+      ctypes.POINTER(ctypes.c_float)]),
+    ("wt_autotune_save", ctypes.c_int, [_vp, ctypes.c_char_p]),
+    ("wt_autotune_load", ctypes.c_int, [_vp, ctypes.c_char_p]),
+    ("wt_graph_begin", ctypes.c_int, [_vp, _vp]),
+    ("wt_graph_end", ctypes.c_int, [_vp, _vp, ctypes.POINTER(_vp)]),
+    ("wt_graph_launch", ctypes.c_int, [_vp, _vp]),
+    ("wt_graph_destroy", None, [_vp]),
+]
+
+_lib = None
+
+
+class WtError(RuntimeError):
+    def __init__(self, status, msg=""):
+        self.status = status
+        super().__init__("wt status %d (%s)%s" % (status, STATUS_NAMES.get(status, "?"), (": " + msg) if msg else ""))
+
+
+STATUS_NAMES = {0: "ok", 1: "invalid argument", 2: "no intersection", 3: "HIP error", 4: "out of memory",
+                5: "unsupported", 6: "degenerate"}
+
+
+def lib(load_torch=True):
+    """Load libwavetool.so. Raises if the library has not been built: there is no
+    fallback path."""
+    global _lib
+    if _lib is None:
+        if load_torch:
+            import torch  # noqa: F401  (binds the HIP runtime torch uses first)
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libwavetool.so is missing (%s); run __graft_entry__.build()" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in PROTOTYPES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status, ctx=None):
+    if status != WT_OK:
+        msg = ""
+        if ctx is not None:
+            m = lib().wt_last_error(ctx)
+            msg = m.decode() if m else ""
+        raise WtError(status, msg)

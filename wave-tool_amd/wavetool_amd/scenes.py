@@ -1,0 +1,223 @@
+"""Seeded scene generators for the water-displacement workloads.
+
+All randomness comes from SplitMix64 with u = (x >> 11) * 2^-53, so every draw is
+reproducible across platforms (std / numpy distributions are not). The parameter
+mapping onto the reference's wave struct (src/render-engine.h:130-167,
+assets/shaders/water-grid.vert:45-51):
+  frequency_w        w   = 2 pi / L                  (wavenumber)
+  phaseConstant_phi  phi = sqrt(g 2 pi / L)          (deep-water angular speed,
+                                                      the reference's phi = S w)
+  phaseOffset_phi0   phi0 ~ U[0, 2 pi)
+  steepness_Q_i      Q_i = q / (w A Count)           (render-engine.cpp:1073,
+                                                      Count = normaliser)
+so the kernel phase w D.x + phi t + phi0 is the reference's w D.x + phi t plus the
+drawn offset.
+"""
+import math
+
+import numpy as np
+
+MASK = (1 << 64) - 1
+GAMMA = 0x9E3779B97F4A7C15
+G = 9.81
+
+
+class SplitMix64:
+    def __init__(self, seed):
+        self.state = seed & MASK
+
+    def next_u64(self):
+        self.state = (self.state + GAMMA) & MASK
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK
+        return z ^ (z >> 31)
+
+    def uniform(self, lo=0.0, hi=1.0):
+        return lo + (hi - lo) * ((self.next_u64() >> 11) * (1.0 / 9007199254740992.0))
+
+    def randint(self, lo, hi):
+        """Uniform integer in [lo, hi] (inclusive)."""
+        return lo + int(self.uniform() * (hi - lo + 1)) if hi > lo else lo
+
+    def normal(self):
+        u1 = max(self.uniform(), 1e-300)
+        u2 = self.uniform()
+        return math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u2)
+
+
+def splitmix64_array(seed, n):
+    """The first n outputs of SplitMix64(seed), vectorised (uint64 wraps mod 2^64)."""
+    with np.errstate(over="ignore"):
+        i = np.arange(1, n + 1, dtype=np.uint64)
+        z = np.uint64(seed & MASK) + i * np.uint64(GAMMA)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform_array(seed, n):
+    return (splitmix64_array(seed, n) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def draw_wave_params(rng, W, wind_deg=0.0):
+    """Per wave (L, A, angle_rad, phi0, q) drawn as: L ~ U[1,60] m, A = L / U[60,120],
+    angle = wind + U[-45, 45] deg, phi0 ~ U[0, 2 pi), q ~ U[0, 1]."""
+    out = []
+    for _ in range(W):
+        L = rng.uniform(1.0, 60.0)
+        A = L / rng.uniform(60.0, 120.0)
+        ang = math.radians(wind_deg + rng.uniform(-45.0, 45.0))
+        phi0 = rng.uniform(0.0, 2.0 * math.pi)
+        q = rng.uniform(0.0, 1.0)
+        out.append((L, A, ang, phi0, q))
+    return out
+
+
+def waves_from_params(params, normaliser):
+    """float32 array (W, 8) in wt_gerstner_wave layout {A, w, phi, Q_i, Dx, Dz, phi0, 0}."""
+    W = len(params)
+    arr = np.zeros((W, 8), dtype=np.float32)
+    for i, (L, A, ang, phi0, q) in enumerate(params):
+        w = np.float32(2.0 * math.pi / L)
+        a = np.float32(A)
+        arr[i, 0] = a
+        arr[i, 1] = w
+        arr[i, 2] = np.float32(math.sqrt(G * 2.0 * math.pi / L))
+        wa = np.float32(w * a)
+        # render-engine.cpp:1073, evaluated in FP32
+        arr[i, 3] = np.float32(np.float32(q) / np.float32(wa * np.float32(normaliser))) if wa != 0 else np.float32(0)
+        arr[i, 4] = np.float32(math.cos(ang))
+        arr[i, 5] = np.float32(math.sin(ang))
+        arr[i, 6] = np.float32(phi0)
+    return arr
+
+
+def gen_waves(seed, W, normaliser=None, wind_deg=0.0):
+    rng = SplitMix64(seed)
+    return waves_from_params(draw_wave_params(rng, W, wind_deg), W if normaliser is None else normaliser)
+
+
+def value_noise(seed, size=512, octaves=8, base_cells=4):
+    """Tiling value noise in [0, 1]: octave o has base_cells * 2^o lattice cells per
+    side (period = size texels), lattice values U[0,1) from SplitMix64, smoothstep
+    interpolation at texel centres, amplitudes 2^-o normalised to sum 1."""
+    out = np.zeros((size, size), dtype=np.float64)
+    total = 0.0
+    centres = (np.arange(size) + 0.5) / size
+    for o in range(octaves):
+        cells = base_cells * (1 << o)
+        lat = uniform_array(seed * 1000003 + o, cells * cells).reshape(cells, cells)
+        f = centres * cells
+        i0 = np.floor(f).astype(np.int64) % cells
+        i1 = (i0 + 1) % cells
+        t = f - np.floor(f)
+        t = t * t * (3.0 - 2.0 * t)
+        # rows (j) then columns (i)
+        a = lat[np.ix_(i0, i0)] * (1 - t)[None, :] + lat[np.ix_(i0, i1)] * t[None, :]
+        b = lat[np.ix_(i1, i0)] * (1 - t)[None, :] + lat[np.ix_(i1, i1)] * t[None, :]
+        v = a * (1 - t)[:, None] + b * t[:, None]
+        amp = 0.5 ** o
+        out += amp * v
+        total += amp
+    return np.clip(out / total, 0.0, 1.0).astype(np.float32)
+
+
+# --------------------------------------------------------------------- workloads
+def camera_tuple(height=10.0, pitch_deg=-15.0, yaw_deg=-90.0, fov_deg=60.0, aspect=16.0 / 9.0, near=0.1, far=1000.0,
+                 x=0.0, z=0.0):
+    """(fov, aspect, near, far, px, py, pz, yaw, pitch), float32-representable values."""
+    f = lambda v: float(np.float32(v))  # noqa: E731
+    return tuple(f(v) for v in (fov_deg, aspect, near, far, x, height, z, yaw_deg, pitch_deg))
+
+
+DISPLACEMENT_BOUND = 2.0  # projector slab half-height (BASELINE configs 0 and 1)
+
+
+def plain_config(index):
+    """Configs 0 (CPU, 256^2 x 4 waves, t = 10 s) and 1 (GPU, 1024^2 x 64 waves + 512^2
+    value-noise heightmap, 240 frames at t = 10 + k/60)."""
+    if index == 0:
+        return dict(N=256, W=4, seed=0, frames=1, t0=10.0, dt=0.0, heightmap=None, camera=camera_tuple(),
+                    sample_scale=0.0, displacement_scale=0.0, scroll=(0.0, 0.0), amp=DISPLACEMENT_BOUND,
+                    with_tb=True)
+    if index == 1:
+        # heightmap: 512^2 R32F value noise, x 0.05 m, scrolled 0.3 m/s along the wind
+        # (angle 0 -> +x), sampled with REPEAT at s_h = 0.02 (the reference default,
+        # render-engine.h:225): one texture tile spans 50 m.
+        return dict(N=1024, W=64, seed=1, frames=240, t0=10.0, dt=1.0 / 60.0, heightmap=dict(seed=1, size=512),
+                    camera=camera_tuple(), sample_scale=0.02, displacement_scale=0.05, scroll=(0.3, 0.0),
+                    amp=DISPLACEMENT_BOUND, with_tb=False)
+    raise ValueError(index)
+
+
+def batched_patches(seed=2, count=512):
+    """Config 2: per patch grid side log-uniform in [32, 1024], W ~ U{4..128}, camera
+    height U[2,50] m, pitch U[-35,-5] deg, yaw U[0,360) deg, t ~ U[0, 3600) s."""
+    rng = SplitMix64(seed)
+    out = []
+    for p in range(count):
+        N = int(round(math.exp(rng.uniform(math.log(32.0), math.log(1024.0)))))
+        W = rng.randint(4, 128)
+        h = rng.uniform(2.0, 50.0)
+        pitch = rng.uniform(-35.0, -5.0)
+        yaw = rng.uniform(0.0, 360.0)
+        t = rng.uniform(0.0, 3600.0)
+        out.append(dict(N=N, W=W, camera=camera_tuple(height=h, pitch_deg=pitch, yaw_deg=yaw), time=t,
+                        wave_seed=seed * 100003 + p))
+    return out
+
+
+def fit_problem(seed, n, S, W_hidden, W_fit, spacing=0.5, dt=0.25, noise=0.01):
+    """Configs 3/4: hidden waves (drawn as config 0, normalised by their count), target
+    displacement on an n x n world grid (centred on the origin) at S snapshots
+    t = s * dt, plus N(0, noise) on every component; a perturbed initial guess
+    (every parameter x U[0.8, 1.2], direction + U[-15, 15] deg).
+    Parameters per wave: (A, L, alpha, Q, phi0)."""
+    rng = SplitMix64(seed)
+    hid = draw_wave_params(rng, W_hidden)
+    hw = waves_from_params(hid, W_hidden)
+    hidden = np.array([(float(hw[i, 0]), L, ang, float(hw[i, 3]), phi0)
+                       for i, (L, A, ang, phi0, q) in enumerate(hid)], dtype=np.float64)
+    fit_src = hidden[:W_fit] if W_fit <= W_hidden else np.concatenate(
+        [hidden, hidden[: W_fit - W_hidden]], axis=0)
+    init = fit_src.copy()
+    for i in range(init.shape[0]):
+        for k in (0, 1, 3, 4):
+            init[i, k] *= rng.uniform(0.8, 1.2)
+        init[i, 2] += math.radians(rng.uniform(-15.0, 15.0))
+    x0 = -0.5 * (n - 1) * spacing
+    return dict(n=n, S=S, spacing=spacing, x0=x0, z0=x0, t0=0.0, dt=dt, noise=noise, hidden=hidden,
+                init=init, noise_seed=seed * 7919 + 1)
+
+
+def fit_target(prob, chunk=1 << 16):
+    """Target displacement [S][n*n][3] float32 of the hidden waves plus noise, in FP64."""
+    n, S = prob["n"], prob["S"]
+    V = n * n
+    h = prob["hidden"]
+    A, L, al, Q, ph0 = (h[:, k] for k in range(5))
+    w = 2.0 * math.pi / L
+    om = np.sqrt(G * w)
+    dx, dz = np.cos(al), np.sin(al)
+    idx = np.arange(V)
+    xs = prob["x0"] + (idx % n) * prob["spacing"]
+    zs = prob["z0"] + (idx // n) * prob["spacing"]
+    out = np.zeros((S, V, 3), dtype=np.float32)
+    for s in range(S):
+        t = prob["t0"] + s * prob["dt"]
+        for lo in range(0, V, chunk):
+            x = xs[lo:lo + chunk, None]
+            z = zs[lo:lo + chunk, None]
+            th = w[None, :] * (dx[None, :] * x + dz[None, :] * z) + om[None, :] * t + ph0[None, :]
+            c, sn = np.cos(th), np.sin(th)
+            out[s, lo:lo + chunk, 0] = (c * (Q * A * dx)[None, :]).sum(1)
+            out[s, lo:lo + chunk, 1] = (sn * A[None, :]).sum(1)
+            out[s, lo:lo + chunk, 2] = (c * (Q * A * dz)[None, :]).sum(1)
+    if prob["noise"] > 0:
+        u1 = np.maximum(uniform_array(prob["noise_seed"], S * V * 3), 1e-300)
+        u2 = uniform_array(prob["noise_seed"] + 1, S * V * 3)
+        nz = np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * math.pi * u2)
+        out += (prob["noise"] * nz).reshape(S, V, 3).astype(np.float32)
+    return out
