@@ -134,12 +134,12 @@ typedef struct wt_vertex_buffers {
 } wt_vertex_buffers;
 
 /* Kernel launch shape for the forward kernels; 0 in a field means "autotuned
- * or default". variant: 1 = VALU accumulation, 2 = MFMA 4x4x1 accumulation. */
+ * or default". */
 typedef struct wt_launch_config {
-  uint32_t variant;
-  uint32_t steps_per_lane; /* columns each lane advances by the sin/cos recurrence: 4, 8 or 16 */
+  uint32_t wave_split;     /* wavefronts sharing one block of vertices, each summing 1/wave_split of the waves: 1 or 2 */
+  uint32_t steps_per_lane; /* vertices each lane advances along its row by the sin/cos recurrence: 4, 8 or 16 */
   uint32_t lanes_per_row;  /* lanes of a 64-wide wavefront sharing one grid row: 64, 32, 16 or 8 */
-  uint32_t wave_chunk;     /* waves staged in LDS per pass: 32, 64 or 128 */
+  uint32_t wave_chunk;     /* waves staged in LDS per pass (64 in this build) */
 } wt_launch_config;
 
 /* ---------------------------------------------------------------- context */

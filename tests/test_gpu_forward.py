@@ -101,17 +101,17 @@ def test_config1_frame_parity(torch_cuda, wtctx, frame):
 SHAPES = [(v, k, lc) for v in (1, 2) for k in (4, 8, 16) for lc in (64, 32, 16, 8)]
 
 
-@pytest.mark.parametrize("variant,steps,lanes", SHAPES)
-def test_launch_shapes(torch_cuda, wtctx, variant, steps, lanes):
+@pytest.mark.parametrize("split,steps,lanes", SHAPES)
+def test_launch_shapes(torch_cuda, wtctx, split, steps, lanes):
     """Every launch shape on a ragged size (N = 301 is no multiple of any segment
     width) and a wave count that spans two LDS chunks (W = 70)."""
-    say("shape variant=%d steps=%d lanes=%d N=301 W=70" % (variant, steps, lanes))
+    say("shape split=%d steps=%d lanes=%d N=301 W=70" % (split, steps, lanes))
     cam = scenes.camera_tuple(height=6.0, pitch_deg=-25.0, yaw_deg=30.0)
     corners = wt.projector_corners(cam, 2.0)
     waves = scenes.gen_waves(11, 70)
     texels = scenes.value_noise(5, 64)
     cfg = dict(N=301, sample_scale=0.05, displacement_scale=0.1, scroll=(0.2, -0.1))
-    wtctx.set_launch_config(variant, steps, lanes, 64)
+    wtctx.set_launch_config(split, steps, lanes, 64)
     try:
         res = check_parity(torch_cuda, wtctx, cfg, corners, waves, cam[4:7], 37.25, texels, True)
     finally:

@@ -51,12 +51,13 @@ int bucket_log2(uint32_t v) {
 bool valid_shape(const wt::LaunchShape& s) {
   const bool k_ok = s.steps == 4 || s.steps == 8 || s.steps == 16;
   const bool lc_ok = s.lanes_per_row == 64 || s.lanes_per_row == 32 || s.lanes_per_row == 16 || s.lanes_per_row == 8;
-  return (s.variant == 1 || s.variant == 2) && k_ok && lc_ok && (s.wave_chunk == 64 || s.wave_chunk == 0);
+  return (s.split == 1 || s.split == 2) && k_ok && lc_ok && (s.wave_chunk == 64 || s.wave_chunk == 0) &&
+         s.steps >= 2 * (s.split - 1);
 }
 
 wt::LaunchShape default_shape(uint32_t N) {
   // lanes per row no wider than the row, steps so one segment spans about a row
-  wt::LaunchShape s{1, 8, 64, 64};
+  wt::LaunchShape s{2, 8, 64, 64};
   if (N <= 64) s.lanes_per_row = 16;
   else if (N <= 256) s.lanes_per_row = 32;
   if (N <= 64) s.steps = 4;
@@ -274,11 +275,11 @@ wt_status wt_water_displace(wt_ctx* ctx, const wt_water_params* p, const wt_gers
 
 wt_status wt_set_launch_config(wt_ctx* ctx, const wt_launch_config* cfg) {
   if (!ctx || !cfg) return WT_ERR_INVALID_ARG;
-  if (!cfg->variant && !cfg->steps_per_lane && !cfg->lanes_per_row && !cfg->wave_chunk) {
+  if (!cfg->wave_split && !cfg->steps_per_lane && !cfg->lanes_per_row && !cfg->wave_chunk) {
     ctx->forced = false;
     return WT_OK;
   }
-  wt::LaunchShape s{(int)cfg->variant, (int)cfg->steps_per_lane, (int)cfg->lanes_per_row,
+  wt::LaunchShape s{(int)cfg->wave_split, (int)cfg->steps_per_lane, (int)cfg->lanes_per_row,
                     cfg->wave_chunk ? (int)cfg->wave_chunk : 64};
   if (!valid_shape(s)) return fail(ctx, WT_ERR_INVALID_ARG, "unsupported launch configuration");
   ctx->forced = true;
@@ -289,7 +290,7 @@ wt_status wt_set_launch_config(wt_ctx* ctx, const wt_launch_config* cfg) {
 wt_status wt_get_launch_config(wt_ctx* ctx, uint32_t N, uint32_t W, wt_launch_config* cfg) {
   if (!ctx || !cfg) return WT_ERR_INVALID_ARG;
   const wt::LaunchShape s = pick_shape(ctx, N, W);
-  cfg->variant = (uint32_t)s.variant;
+  cfg->wave_split = (uint32_t)s.split;
   cfg->steps_per_lane = (uint32_t)s.steps;
   cfg->lanes_per_row = (uint32_t)s.lanes_per_row;
   cfg->wave_chunk = (uint32_t)(s.wave_chunk ? s.wave_chunk : 64);
@@ -357,8 +358,8 @@ wt_status wt_autotune_forward(wt_ctx* ctx, uint32_t N, uint32_t W, int with_hm, 
   const wt::LaunchShape saved_shape = ctx->forced_shape;
   float best_t = 1e30f;
   wt::LaunchShape best_s = default_shape(N);
-  const int vars[2] = {1, 2}, ks[3] = {4, 8, 16}, lcs[4] = {64, 32, 16, 8};
-  for (int var : vars)
+  const int splits[2] = {1, 2}, ks[3] = {4, 8, 16}, lcs[4] = {64, 32, 16, 8};
+  for (int var : splits)
     for (int k : ks)
       for (int lc : lcs) {
         if ((uint32_t)lc > N && lc != 8) continue;
@@ -393,7 +394,7 @@ wt_status wt_autotune_forward(wt_ctx* ctx, uint32_t N, uint32_t W, int with_hm, 
   ctx->forced_shape = saved_shape;
   ctx->table[{bucket_log2(N), bucket_log2(W ? W : 1)}] = best_s;
   if (best) {
-    best->variant = (uint32_t)best_s.variant;
+    best->wave_split = (uint32_t)best_s.split;
     best->steps_per_lane = (uint32_t)best_s.steps;
     best->lanes_per_row = (uint32_t)best_s.lanes_per_row;
     best->wave_chunk = 64;
@@ -407,9 +408,9 @@ wt_status wt_autotune_save(wt_ctx* ctx, const char* path) {
   if (!ctx || !path) return WT_ERR_INVALID_ARG;
   FILE* f = std::fopen(path, "w");
   if (!f) return fail(ctx, WT_ERR_INVALID_ARG, std::string("cannot open ") + path);
-  std::fprintf(f, "# wt forward autotune: n_bucket w_bucket variant steps lanes_per_row wave_chunk\n");
+  std::fprintf(f, "# wt forward autotune: n_bucket w_bucket wave_split steps lanes_per_row wave_chunk\n");
   for (const auto& kv : ctx->table)
-    std::fprintf(f, "%d %d %d %d %d %d\n", kv.first.first, kv.first.second, kv.second.variant, kv.second.steps,
+    std::fprintf(f, "%d %d %d %d %d %d\n", kv.first.first, kv.first.second, kv.second.split, kv.second.steps,
                  kv.second.lanes_per_row, kv.second.wave_chunk);
   std::fclose(f);
   return WT_OK;
@@ -425,7 +426,7 @@ wt_status wt_autotune_load(wt_ctx* ctx, const char* path) {
     if (line[0] == '#') continue;
     int nb, wb;
     wt::LaunchShape s{};
-    if (std::sscanf(line, "%d %d %d %d %d %d", &nb, &wb, &s.variant, &s.steps, &s.lanes_per_row, &s.wave_chunk) != 6 ||
+    if (std::sscanf(line, "%d %d %d %d %d %d", &nb, &wb, &s.split, &s.steps, &s.lanes_per_row, &s.wave_chunk) != 6 ||
         !valid_shape(s)) {
       std::fclose(f);
       return fail(ctx, WT_ERR_INVALID_ARG, "malformed autotune table");

@@ -45,25 +45,25 @@ __device__ __forceinline__ void stage_wave(const wt_gerstner_wave& g, double t, 
   o->cs[3] = qaw * Dz * Dz;
 }
 
-// Bilinear heightmap tap with GL texel-centre and wrap rules. (s, t) are texture
-// coordinates; returns the texel-space value and dH/ds, dH/dt of the same cell.
-// Index arithmetic is clamped so any finite or non-finite input stays in bounds.
-__device__ __forceinline__ float heightmap_tap(const void* __restrict__ texels, int W, int H, int fmt, int wrap_mode,
-                                               float s, float t, float* dds, float* ddt) {
+// Bilinear heightmap tap with GL texel-centre and wrap rules, split in two so a lane can
+// issue the texel loads of all its vertices before it needs any of them.
+// heightmap_fetch: (s, t) are texture coordinates already reduced to one wrap period
+// ([0, 1) for REPEAT, [0, 2) for MIRRORED_REPEAT); index arithmetic is clamped so any
+// input stays in bounds. heightmap_eval: value and dH/ds, dH/dt of the same cell.
+struct HmTap {
+  float t00, t10, t01, t11, a, b;
+};
+
+__device__ __forceinline__ HmTap heightmap_fetch(const void* __restrict__ texels, int W, int H, int fmt, int wrap_mode,
+                                                 float s, float t) {
   const bool repeat = wrap_mode == WT_WRAP_REPEAT;
-  // reduce to one period first so the texel coordinate stays small
-  if (repeat) {
-    s = s - floorf(s);
-    t = t - floorf(t);
-  } else {
-    s = s - 2.0f * floorf(0.5f * s);
-    t = t - 2.0f * floorf(0.5f * t);
-  }
   const int period_w = repeat ? W : 2 * W;
   const int period_h = repeat ? H : 2 * H;
   const float u = s * (float)W - 0.5f, v = t * (float)H - 0.5f;
   const float fu = floorf(u), fv = floorf(v);
-  const float a = u - fu, b = v - fv;
+  HmTap h;
+  h.a = u - fu;
+  h.b = v - fv;
   // fmaxf/fminf return the non-NaN operand, so a NaN coordinate lands on texel -1
   int i0 = (int)fminf(fmaxf(fu, -1.0f), (float)(period_w - 1));
   int j0 = (int)fminf(fmaxf(fv, -1.0f), (float)(period_h - 1));
@@ -77,29 +77,37 @@ __device__ __forceinline__ float heightmap_tap(const void* __restrict__ texels, 
   i1 = wrap(i1, W, period_w);
   j0 = wrap(j0, H, period_h);
   j1 = wrap(j1, H, period_h);
-  float t00, t10, t01, t11;
   if (fmt == WT_TEXEL_R8_UNORM) {
     const uint8_t* p = (const uint8_t*)texels;
-    const float k = 1.0f / 255.0f;
-    t00 = (float)p[j0 * W + i0] * k;
-    t10 = (float)p[j0 * W + i1] * k;
-    t01 = (float)p[j1 * W + i0] * k;
-    t11 = (float)p[j1 * W + i1] * k;
+    h.t00 = (float)p[j0 * W + i0];
+    h.t10 = (float)p[j0 * W + i1];
+    h.t01 = (float)p[j1 * W + i0];
+    h.t11 = (float)p[j1 * W + i1];
   } else {
     const float* p = (const float*)texels;
-    t00 = p[j0 * W + i0];
-    t10 = p[j0 * W + i1];
-    t01 = p[j1 * W + i0];
-    t11 = p[j1 * W + i1];
+    h.t00 = p[j0 * W + i0];
+    h.t10 = p[j0 * W + i1];
+    h.t01 = p[j1 * W + i0];
+    h.t11 = p[j1 * W + i1];
   }
-  const float lo = fmaf(a, t10 - t00, t00), hi = fmaf(a, t11 - t01, t01);
-  *dds = (float)W * fmaf(b, (t11 - t01) - (t10 - t00), t10 - t00);
-  *ddt = (float)H * fmaf(a, (t11 - t10) - (t01 - t00), t01 - t00);
-  return fmaf(b, hi - lo, lo);
+  return h;
 }
 
-// Per-vertex epilogue: assemble position and partials, add the heightmap bump at
-// the displaced point and the bounce, form N / T / B, store.
+__device__ __forceinline__ float heightmap_eval(HmTap h, int W, int H, int fmt, float* dds, float* ddt) {
+  if (fmt == WT_TEXEL_R8_UNORM) {  // unorm8: c / 255
+    const float k = 1.0f / 255.0f;
+    h.t00 *= k; h.t10 *= k; h.t01 *= k; h.t11 *= k;
+  }
+  const float lo = fmaf(h.a, h.t10 - h.t00, h.t00), hi = fmaf(h.a, h.t11 - h.t01, h.t01);
+  *dds = (float)W * fmaf(h.b, (h.t11 - h.t01) - (h.t10 - h.t00), h.t10 - h.t00);
+  *ddt = (float)H * fmaf(h.a, (h.t11 - h.t10) - (h.t01 - h.t00), h.t01 - h.t00);
+  return fmaf(h.b, hi - lo, lo);
+}
+
+// Per-vertex epilogue, in two phases so the texel loads of all of a lane's vertices are
+// in flight together: vertex_fetch forms the displaced point and issues the heightmap
+// loads; vertex_finish adds the bump at the displaced point and the bounce, forms
+// N / T / B and stores.
 struct EpilogueArgs {
   float cam[3];
   float s_h, s_d, bounce;
@@ -112,24 +120,22 @@ struct EpilogueArgs {
   float* bit;
 };
 
+struct VertexPending {
+  float px, pz;
+  HmTap tap;
+};
+
 // acc: X, Z, Cx, Cz (cos sums) and Y, Sxx, Sxz, Szz (sin sums).
 template <int kHm>
-__device__ __forceinline__ void vertex_epilogue(const EpilogueArgs& e, double x, double z, const float acc[8],
-                                                size_t vid) {
-  const float X = acc[0], Z = acc[1], Cx = acc[2], Cz = acc[3];
-  const float Y = acc[4], Sxx = acc[5], Sxz = acc[6], Szz = acc[7];
-  const double pxd = x + (double)X, pzd = z + (double)Z;
-  const float px = (float)pxd, pz = (float)pzd;
-  float py = Y;
-  // dP/dx and dP/dz of the Gerstner surface (no reference counterpart: the
-  // reference differences neighbours, water-grid.vert:164-187)
-  float tx = 1.0f - Sxx, ty = Cx, tz = -Sxz;
-  float bx = -Sxz, by = Cz, bz = 1.0f - Szz;
+__device__ __forceinline__ VertexPending vertex_fetch(const EpilogueArgs& e, double x, double z, const float acc[8]) {
+  VertexPending v;
+  const double pxd = x + (double)acc[0], pzd = z + (double)acc[1];
+  v.px = (float)pxd;
+  v.pz = (float)pzd;
   if (kHm) {
     // water-grid.vert:103-110: sample at s_h * (x, -z) of the displaced point. The
     // texture coordinate is formed from the FP64 displaced point and reduced to one
     // wrap period in FP64, so its FP32 rounding is relative to a number below 2.
-    float dds, ddt;
     double sd = fma((double)e.s_h, pxd, e.hm_off_s), td = fma(-(double)e.s_h, pzd, e.hm_off_t);
     if (e.tex_wrap == WT_WRAP_REPEAT) {
       sd -= floor(sd);
@@ -138,9 +144,26 @@ __device__ __forceinline__ void vertex_epilogue(const EpilogueArgs& e, double x,
       sd -= 2.0 * floor(0.5 * sd);
       td -= 2.0 * floor(0.5 * td);
     }
-    const float s = (float)sd, t = (float)td;
-    const float r = heightmap_tap(e.texels, e.tex_w, e.tex_h, e.tex_fmt, e.tex_wrap, s, t, &dds, &ddt);
-    py += e.s_d * (2.0f * (r - 0.5f));
+    v.tap = heightmap_fetch(e.texels, e.tex_w, e.tex_h, e.tex_fmt, e.tex_wrap, (float)sd, (float)td);
+  }
+  return v;
+}
+
+template <int kHm>
+__device__ __forceinline__ void vertex_finish(const EpilogueArgs& e, const VertexPending& v, const float acc[8],
+                                              size_t vid) {
+  const float Cx = acc[2], Cz = acc[3];
+  const float Y = acc[4], Sxx = acc[5], Sxz = acc[6], Szz = acc[7];
+  const float px = v.px, pz = v.pz;
+  float py = Y;
+  // dP/dx and dP/dz of the Gerstner surface (no reference counterpart: the
+  // reference differences neighbours, water-grid.vert:164-187)
+  float tx = 1.0f - Sxx, ty = Cx, tz = -Sxz;
+  float bx = -Sxz, by = Cz, bz = 1.0f - Szz;
+  if (kHm) {
+    float dds, ddt;
+    const float r = heightmap_eval(v.tap, e.tex_w, e.tex_h, e.tex_fmt, &dds, &ddt);
+    py += e.s_d * (2.0f * (r - 0.5f));  // water-grid.vert:108-110
     const float k = 2.0f * e.s_d * e.s_h;
     const float hpx = k * dds, hpz = -k * ddt;  // d(bump)/dPx, d(bump)/dPz
     ty += hpx * tx + hpz * tz;
@@ -149,7 +172,7 @@ __device__ __forceinline__ void vertex_epilogue(const EpilogueArgs& e, double x,
   py += e.bounce;  // water-grid.vert:147
   // N = normalize(B x T): y-up for the undisplaced plane
   float nx = by * tz - bz * ty, ny = bz * tx - bx * tz, nz = bx * ty - by * tx;
-  const float inv_n = 1.0f / sqrtf(nx * nx + ny * ny + nz * nz);
+  const float inv_n = __builtin_amdgcn_rsqf(nx * nx + ny * ny + nz * nz);
   nx *= inv_n; ny *= inv_n; nz *= inv_n;
   // flip toward the camera (water-grid.vert:190)
   const float vx = e.cam[0] - px, vy = e.cam[1] - py, vz = e.cam[2] - pz;
@@ -159,12 +182,12 @@ __device__ __forceinline__ void vertex_epilogue(const EpilogueArgs& e, double x,
   float* Nn = e.nrm + 3 * vid;
   Nn[0] = nx; Nn[1] = ny; Nn[2] = nz;
   if (e.tan) {
-    const float it = 1.0f / sqrtf(tx * tx + ty * ty + tz * tz);
+    const float it = __builtin_amdgcn_rsqf(tx * tx + ty * ty + tz * tz);
     float* Tt = e.tan + 3 * vid;
     Tt[0] = tx * it; Tt[1] = ty * it; Tt[2] = tz * it;
   }
   if (e.bit) {
-    const float ib = 1.0f / sqrtf(bx * bx + by * by + bz * bz);
+    const float ib = __builtin_amdgcn_rsqf(bx * bx + by * by + bz * bz);
     float* Bb = e.bit + 3 * vid;
     Bb[0] = bx * ib; Bb[1] = by * ib; Bb[2] = bz * ib;
   }

@@ -2,23 +2,27 @@
 // over a gridLength^2 projected grid, with analytic normal / tangent / bitangent.
 //
 // Work decomposition (gfx950, wave64):
-//  * A wavefront owns a "segment": LR = 64/LC grid rows x (LC*K) columns. Lane l sits
-//    on row l/LC and column c0 + (l % LC); its k-th vertex is LC*k columns further.
-//    Consecutive lanes hold consecutive columns, so every store instruction writes a
-//    contiguous run of float3 (coalesced).
+//  * A "segment" is LR = 64/LC grid rows x (LC*K) columns. Lane l sits on row l/LC and
+//    column c0 + (l % LC); its k-th vertex is LC*k columns further. Consecutive lanes
+//    hold consecutive columns, so every store instruction writes a contiguous run of
+//    float3 (coalesced).
+//  * S wavefronts share one segment: wavefront s of the group sums waves s, s+S, ...
+//    The S partial sums are exchanged through LDS at the end and each wavefront runs
+//    the epilogue for K/S of the K vertices. S = 2 doubles the number of resident
+//    wavefronts for the same per-lane work, which is what lets gfx950 dual-issue VALU
+//    instructions (two waves per SIMD per quad-cycle).
 //  * Along a row the grid point is affine in the column (bilinear corners), so each
 //    wave's phase advances by a constant angle per LC columns. A lane seeds
-//    (cos, sin) once per wave from an FP64-reduced phase (native v_sin/v_cos on an
-//    argument in [-1/2, 1/2) revolution) and steps K vertices with an FP32 rotation
-//    (4 VALU ops per vertex-wave instead of two transcendentals). The rotation angles
-//    are computed per (row, wave) in FP64 and rounded once.
+//    (cos, sin) once per wave from an FP64-reduced phase (native v_sin / v_cos on an
+//    argument in [0, 1] revolution) and steps K vertices with an FP32 rotation (4 VALU
+//    ops per vertex-wave instead of two transcendentals). The rotation per (row, wave)
+//    is computed in FP64 and rounded once.
 //  * Per-wave constants are staged in LDS per chunk of kChunk waves (WaveLds), the
-//    per-row rotations in a per-wavefront LDS table.
-//  * The 8 running sums per vertex (x/z displacement, y, and the 5 distinct entries
-//    of the analytic Jacobian) are accumulated either with VALU FMAs (variant 1) or
-//    on the matrix cores with v_mfma_f32_4x4x1_16b_f32 (variant 2): 16 independent
-//    4x4 outer products per instruction = (4 vertices x 4 coefficients) per block,
-//    cos-weighted sums in one accumulator set and sin-weighted sums in another.
+//    per-row rotations in a per-wavefront LDS table; the next wave's constants are
+//    loaded while the current wave is summed.
+//  * 8 running sums per vertex: x/z displacement, y, and the 5 distinct entries of the
+//    analytic Jacobian, all VALU FMAs with VGPR operands (no SGPR operand, which would
+//    block dual issue).
 #include <hip/hip_runtime.h>
 
 #include "wt_device.hpp"
@@ -28,8 +32,6 @@ namespace wt {
 
 constexpr int kChunk = 64;      // waves staged per LDS pass
 constexpr int kWavesPerWG = 4;  // 4 wavefronts of 64 = 256 threads
-
-typedef float v4f __attribute__((ext_vector_type(4)));
 
 // FP64 row geometry: x(col) = xa + col * xs, z(col) = za + col * zs.
 struct RowGeom {
@@ -50,38 +52,63 @@ __device__ __forceinline__ void stage_chunk(const FwdParams& p, int w0, int nw, 
   for (int t = threadIdx.x; t < nw; t += blockDim.x) stage_wave(p.waves[w0 + t], p.time, &s_wave[t]);
 }
 
-// Rotation per LC columns for each (row of this segment, wave of the chunk), in FP64.
-template <int LC, int LR>
-__device__ __forceinline__ void build_rotations(const FwdParams& p, int row_base, int nw, const WaveLds* s_wave,
-                                                float2* s_rot, int lane) {
-  for (int e = lane; e < LR * nw; e += 64) {
-    const int r = e / nw, i = e - r * nw;
+// Rotation per LC columns for (row of the segment, wave of the chunk), in FP64, for the
+// waves this wavefront sums (i = sid, sid + S, ...).
+template <int LC, int LR, int S>
+__device__ __forceinline__ void build_rotations(const FwdParams& p, int row_base, int nw, int sid,
+                                                const WaveLds* s_wave, float4* s_rot, int lane) {
+  const int nmine = (nw - sid + S - 1) / S;
+  for (int e = lane; e < LR * nmine; e += 64) {
+    const int r = e / nmine, m = e - r * nmine, i = sid + m * S;
     int row = row_base + r;
     row = row < p.N ? row : p.N - 1;
     const RowGeom g = row_geom(p, row);
     const double step = (s_wave[i].kx * g.xs + s_wave[i].kz * g.zs) * (double)LC;
-    double f = step - floor(step);
-    const double ang = f * kTwoPi;
+    const double ang = (step - floor(step)) * kTwoPi;
     double sn, cs;
     sincos(ang, &sn, &cs);
-    s_rot[r * kChunk + i] = make_float2((float)cs, (float)sn);
+    s_rot[r * kChunk + i] = make_float4((float)cs, (float)sn, -(float)sn, 0.0f);
   }
 }
 
-template <int K, int LC, int kVar, int kHm>
+// Epilogue of vertices [JB, JB + KS) of a lane: all texel loads first, then the per-vertex
+// math and stores (JB is a compile-time constant so the sums stay in registers).
+template <int K, int LC, int KS, int JB, int kHm>
+__device__ __forceinline__ void epilogue_steps(const FwdParams& p, const RowGeom& g, const float (&acc)[K][8],
+                                               int col0, int row, bool active) {
+  VertexPending pend[KS];
+#pragma unroll
+  for (int u = 0; u < KS; ++u) {
+    const int col = col0 + (JB + u) * LC;
+    const double x = fma((double)col, g.xs, g.xa), z = fma((double)col, g.zs, g.za);
+    pend[u] = vertex_fetch<kHm>(p.ep, x, z, acc[JB + u]);
+  }
+#pragma unroll
+  for (int u = 0; u < KS; ++u) {
+    const int col = col0 + (JB + u) * LC;
+    if (active && row < p.N && col < p.N)
+      vertex_finish<kHm>(p.ep, pend[u], acc[JB + u], (size_t)row * (size_t)p.N + (size_t)col);
+  }
+}
+
+template <int K, int LC, int S, int kHm>
 __global__ __launch_bounds__(256) void water_forward_kernel(FwdParams p) {
   constexpr int LR = 64 / LC;
+  constexpr int KS = K / S;                       // vertices per lane in the epilogue
+  constexpr int kSegPerWG = kWavesPerWG / S;
   __shared__ WaveLds s_wave[kChunk];
-  __shared__ float2 s_rot[kWavesPerWG][LR * kChunk];
-  __shared__ float s_xpose[kVar == 2 ? kWavesPerWG : 1][kVar == 2 ? 64 * 9 : 1];
+  __shared__ float4 s_rot[kWavesPerWG][LR * kChunk];  // (cos, sin, -sin, 0) per (row, wave)
+  // partial sums handed to the other wavefront(s) of the group: [group][sender][K-KS][8][64]
+  __shared__ float s_xchg[S > 1 ? kSegPerWG : 1][S > 1 ? S : 1][S > 1 ? (K - KS) * 8 * 64 : 1];
 
   const int lane = threadIdx.x & 63;
-  const int wid = threadIdx.x >> 6;
-  const long long gw = (long long)blockIdx.x * kWavesPerWG + wid;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+  const int grp = wid / S, sid = wid - grp * S;
+  const long long seg_global = (long long)blockIdx.x * kSegPerWG + grp;
   const int segs_per_rg = (p.N + LC * K - 1) / (LC * K);
   const int n_rg = (p.N + LR - 1) / LR;
-  const bool active = gw < (long long)segs_per_rg * n_rg;
-  const int seg_id = active ? (int)gw : 0;
+  const bool active = seg_global < (long long)segs_per_rg * n_rg;
+  const int seg_id = active ? (int)seg_global : 0;
   const int rg = seg_id / segs_per_rg, seg = seg_id - rg * segs_per_rg;
   const int row_base = rg * LR;
   const int lr = lane / LC, lc = lane - lr * LC;
@@ -91,125 +118,109 @@ __global__ __launch_bounds__(256) void water_forward_kernel(FwdParams p) {
   const double x0 = fma((double)col0, g.xs, g.xa), z0 = fma((double)col0, g.zs, g.za);
 
   float acc[K][8];
-  v4f accC[kVar == 2 ? K : 1], accS[kVar == 2 ? K : 1];
 #pragma unroll
   for (int j = 0; j < K; ++j)
 #pragma unroll
     for (int q = 0; q < 8; ++q) acc[j][q] = 0.0f;
-  if (kVar == 2) {
-#pragma unroll
-    for (int j = 0; j < K; ++j) {
-      accC[j] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-      accS[j] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-    }
-  }
 
   for (int w0 = 0; w0 < p.W; w0 += kChunk) {
     const int nw = p.W - w0 < kChunk ? p.W - w0 : kChunk;
     __syncthreads();
     stage_chunk(p, w0, nw, s_wave);
     __syncthreads();
-    build_rotations<LC, LR>(p, row_base, nw, s_wave, s_rot[wid], lane);
+    build_rotations<LC, LR, S>(p, row_base, nw, sid, s_wave, s_rot[wid], lane);
     __syncthreads();
-    const float2* rot_row = &s_rot[wid][lr * kChunk];
-    for (int i = 0; i < nw; ++i) {
-      const WaveLds& wv = s_wave[i];
-      const float2 rot = rot_row[i];
-      const float r = reduce_rev(fma(wv.kx, x0, fma(wv.kz, z0, wv.c0)));
+    const float4* rot_row = &s_rot[wid][lr * kChunk];
+    if (sid >= nw) continue;
+    for (int i = sid; i < nw; i += S) {
+      const WaveLds& cur = s_wave[i];
+      const float4 rot = rot_row[i];
+      const float r = reduce_rev(fma(cur.kx, x0, fma(cur.kz, z0, cur.c0)));
       float c = __builtin_amdgcn_cosf(r), s = __builtin_amdgcn_sinf(r);
-      if (kVar == 1) {
-        const float c0 = wv.cc[0], c1 = wv.cc[1], c2 = wv.cc[2], c3 = wv.cc[3];
-        const float s0 = wv.cs[0], s1 = wv.cs[1], s2 = wv.cs[2], s3 = wv.cs[3];
+      const float c0 = cur.cc[0], c1 = cur.cc[1], c2 = cur.cc[2], c3 = cur.cc[3];
+      const float s0 = cur.cs[0], s1 = cur.cs[1], s2 = cur.cs[2], s3 = cur.cs[3];
 #pragma unroll
-        for (int j = 0; j < K; ++j) {
-          acc[j][0] = fmaf(c0, c, acc[j][0]);
-          acc[j][1] = fmaf(c1, c, acc[j][1]);
-          acc[j][2] = fmaf(c2, c, acc[j][2]);
-          acc[j][3] = fmaf(c3, c, acc[j][3]);
-          acc[j][4] = fmaf(s0, s, acc[j][4]);
-          acc[j][5] = fmaf(s1, s, acc[j][5]);
-          acc[j][6] = fmaf(s2, s, acc[j][6]);
-          acc[j][7] = fmaf(s3, s, acc[j][7]);
-          const float cn = fmaf(c, rot.x, -s * rot.y);
-          s = fmaf(s, rot.x, c * rot.y);
-          c = cn;
-        }
-      } else {
-        const float bc = wv.cc[lane & 3], bs = wv.cs[lane & 3];
-#pragma unroll
-        for (int j = 0; j < K; ++j) {
-          accC[j] = __builtin_amdgcn_mfma_f32_4x4x1f32(c, bc, accC[j], 0, 0, 0);
-          accS[j] = __builtin_amdgcn_mfma_f32_4x4x1f32(s, bs, accS[j], 0, 0, 0);
-          const float cn = fmaf(c, rot.x, -s * rot.y);
-          s = fmaf(s, rot.x, c * rot.y);
+      for (int j = 0; j < K; ++j) {
+        acc[j][0] = fmaf(c0, c, acc[j][0]);
+        acc[j][1] = fmaf(c1, c, acc[j][1]);
+        acc[j][2] = fmaf(c2, c, acc[j][2]);
+        acc[j][3] = fmaf(c3, c, acc[j][3]);
+        acc[j][4] = fmaf(s0, s, acc[j][4]);
+        acc[j][5] = fmaf(s1, s, acc[j][5]);
+        acc[j][6] = fmaf(s2, s, acc[j][6]);
+        acc[j][7] = fmaf(s3, s, acc[j][7]);
+        if (j + 1 < K) {
+          // (c, s) <- (c cr - s sr, s cr + c sr); -sr is staged so every op is a plain FMA
+          const float cn = fmaf(s, rot.z, c * rot.x);
+          s = fmaf(c, rot.y, s * rot.x);
           c = cn;
         }
       }
     }
   }
 
-  // MFMA accumulators: register r of lane 4b+k holds output k of the vertex on lane
-  // 4b+r. Transpose through a per-wavefront LDS tile (row stride 9 floats).
-  float* xp = s_xpose[kVar == 2 ? wid : 0];
+  if (S > 1) {
+    // S = 2: wavefront 0 keeps vertices [0, K/2), wavefront 1 keeps [K/2, K); each hands
+    // the other half of its partial sums over through LDS and adds the half it receives
+    __syncthreads();
+    float* mine = s_xchg[grp][sid];
 #pragma unroll
-  for (int j = 0; j < K; ++j) {
-    if (kVar == 2) {
-      const int base = lane & ~3, k = lane & 3;
+    for (int j = 0; j < KS; ++j) {
+      const int jo = sid == 0 ? j + KS : j;  // the step owned by the other wavefront
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        xp[(base + r) * 9 + k] = accC[j][r];
-        xp[(base + r) * 9 + 4 + k] = accS[j][r];
-      }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-#pragma unroll
-      for (int q = 0; q < 8; ++q) acc[j][q] = xp[lane * 9 + q];
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      for (int q = 0; q < 8; ++q) mine[(j * 8 + q) * 64 + lane] = acc[jo][q];
     }
-    const int col = col0 + j * LC;
-    if (active && row < p.N && col < p.N) {
-      const double x = fma((double)col, g.xs, g.xa), z = fma((double)col, g.zs, g.za);
-      vertex_epilogue<kHm>(p.ep, x, z, acc[j], (size_t)row * (size_t)p.N + (size_t)col);
+    __syncthreads();
+    const float* theirs = s_xchg[grp][sid ^ 1];
+#pragma unroll
+    for (int j = 0; j < KS; ++j) {
+      const int jm = sid == 0 ? j : j + KS;  // my step
+#pragma unroll
+      for (int q = 0; q < 8; ++q) acc[jm][q] += theirs[(j * 8 + q) * 64 + lane];
     }
   }
+
+  // epilogue for the KS vertices this wavefront owns (steps [sid*KS, sid*KS + KS))
+  if (S == 1 || sid == 0)
+    epilogue_steps<K, LC, KS, 0, kHm>(p, g, acc, col0, row, active);
+  else
+    epilogue_steps<K, LC, KS, (S > 1 ? KS : 0), kHm>(p, g, acc, col0, row, active);
 }
 
 // ------------------------------------------------------------------ dispatch
-template <int K, int LC, int kVar, int kHm>
+template <int K, int LC, int S, int kHm>
 static hipError_t launch_t(const FwdParams& p, hipStream_t st) {
   constexpr int LR = 64 / LC;
+  constexpr int kSegPerWG = kWavesPerWG / S;
   const long long segs = (long long)((p.N + LC * K - 1) / (LC * K)) * ((p.N + LR - 1) / LR);
-  const long long blocks = (segs + kWavesPerWG - 1) / kWavesPerWG;
-  hipLaunchKernelGGL((water_forward_kernel<K, LC, kVar, kHm>), dim3((unsigned)blocks), dim3(256), 0, st, p);
+  const long long blocks = (segs + kSegPerWG - 1) / kSegPerWG;
+  hipLaunchKernelGGL((water_forward_kernel<K, LC, S, kHm>), dim3((unsigned)blocks), dim3(256), 0, st, p);
   return hipGetLastError();
 }
 
-template <int K, int LC, int kVar>
+template <int K, int LC, int S>
 static hipError_t launch_hm(const FwdParams& p, hipStream_t st) {
-  return p.ep.texels ? launch_t<K, LC, kVar, 1>(p, st) : launch_t<K, LC, kVar, 0>(p, st);
+  return p.ep.texels ? launch_t<K, LC, S, 1>(p, st) : launch_t<K, LC, S, 0>(p, st);
 }
 template <int K, int LC>
-static hipError_t launch_var(const FwdParams& p, int var, hipStream_t st) {
-  return var == 2 ? launch_hm<K, LC, 2>(p, st) : launch_hm<K, LC, 1>(p, st);
+static hipError_t launch_split(const FwdParams& p, int split, hipStream_t st) {
+  return split == 2 ? launch_hm<K, LC, 2>(p, st) : launch_hm<K, LC, 1>(p, st);
 }
 template <int K>
-static hipError_t launch_lc(const FwdParams& p, int lc, int var, hipStream_t st) {
+static hipError_t launch_lc(const FwdParams& p, int lc, int split, hipStream_t st) {
   switch (lc) {
-    case 64: return launch_var<K, 64>(p, var, st);
-    case 32: return launch_var<K, 32>(p, var, st);
-    case 16: return launch_var<K, 16>(p, var, st);
-    default: return launch_var<K, 8>(p, var, st);
+    case 64: return launch_split<K, 64>(p, split, st);
+    case 32: return launch_split<K, 32>(p, split, st);
+    case 16: return launch_split<K, 16>(p, split, st);
+    default: return launch_split<K, 8>(p, split, st);
   }
 }
 
 hipError_t launch_forward(const FwdParams& p, const LaunchShape& s, hipStream_t st) {
   switch (s.steps) {
-    case 4: return launch_lc<4>(p, s.lanes_per_row, s.variant, st);
-    case 8: return launch_lc<8>(p, s.lanes_per_row, s.variant, st);
-    default: return launch_lc<16>(p, s.lanes_per_row, s.variant, st);
+    case 4: return launch_lc<4>(p, s.lanes_per_row, s.split, st);
+    case 8: return launch_lc<8>(p, s.lanes_per_row, s.split, st);
+    default: return launch_lc<16>(p, s.lanes_per_row, s.split, st);
   }
 }
 

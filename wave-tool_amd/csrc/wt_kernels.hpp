@@ -23,7 +23,7 @@ struct FwdParams {
 };
 
 struct LaunchShape {
-  int variant;        // 1 VALU, 2 MFMA
+  int split;          // S: wavefronts sharing a block of vertices, 1 or 2
   int steps;          // K: 4, 8, 16
   int lanes_per_row;  // LC: 64, 32, 16, 8
   int wave_chunk;     // waves per LDS pass (64 in this build)

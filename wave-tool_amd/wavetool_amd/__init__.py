@@ -118,14 +118,14 @@ class Context:
                                      ctypes.byref(hm) if hm is not None else None, ctypes.byref(vb), _stream(stream))
         self._check(st)
 
-    def set_launch_config(self, variant=0, steps=0, lanes_per_row=0, wave_chunk=0):
-        cfg = LaunchConfig(variant, steps, lanes_per_row, wave_chunk)
+    def set_launch_config(self, wave_split=0, steps=0, lanes_per_row=0, wave_chunk=0):
+        cfg = LaunchConfig(wave_split, steps, lanes_per_row, wave_chunk)
         self._check(lib().wt_set_launch_config(self.handle, ctypes.byref(cfg)))
 
     def get_launch_config(self, N, W):
         cfg = LaunchConfig()
         self._check(lib().wt_get_launch_config(self.handle, N, W, ctypes.byref(cfg)))
-        return dict(variant=cfg.variant, steps=cfg.steps_per_lane, lanes_per_row=cfg.lanes_per_row,
+        return dict(wave_split=cfg.wave_split, steps=cfg.steps_per_lane, lanes_per_row=cfg.lanes_per_row,
                     wave_chunk=cfg.wave_chunk)
 
     def autotune(self, N, W, with_heightmap=True):
@@ -133,7 +133,7 @@ class Context:
         ms = ctypes.c_float()
         self._check(lib().wt_autotune_forward(self.handle, N, W, int(bool(with_heightmap)), ctypes.byref(cfg),
                                               ctypes.byref(ms)))
-        return dict(variant=cfg.variant, steps=cfg.steps_per_lane, lanes_per_row=cfg.lanes_per_row,
+        return dict(wave_split=cfg.wave_split, steps=cfg.steps_per_lane, lanes_per_row=cfg.lanes_per_row,
                     wave_chunk=cfg.wave_chunk), ms.value
 
     def graph_begin(self, stream):

@@ -88,7 +88,7 @@ class VertexBuffers(ctypes.Structure):
 
 class LaunchConfig(ctypes.Structure):
     _fields_ = [
-        ("variant", ctypes.c_uint32),
+        ("wave_split", ctypes.c_uint32),
         ("steps_per_lane", ctypes.c_uint32),
         ("lanes_per_row", ctypes.c_uint32),
         ("wave_chunk", ctypes.c_uint32),
