@@ -210,6 +210,60 @@ wt_status wt_graph_end(wt_ctx* ctx, void* stream, wt_graph** out);
 wt_status wt_graph_launch(wt_graph* graph, void* stream);
 void wt_graph_destroy(wt_graph* graph);
 
+/* ---------------------------------------------------------------- heightfield fit */
+/* Fitting a wave set to observed displacements (no reference counterpart: the
+ * reference only evaluates waves, water-grid.vert:74-85). Model per wave
+ * {A, L, alpha, Q, phi0}: w = 2 pi / L, omega = sqrt(g w), D = (cos alpha,
+ * sin alpha), theta = w D.x + omega t + phi0 (the reference's phase,
+ * water-grid.vert:78), displacement (Q A Dx cos, A sin, Q A Dz cos)
+ * (water-grid.vert:79-81). Loss = 1/(2 S V) sum |displacement - target|^2 over
+ * an n x n world grid and S snapshots. */
+typedef struct wt_fit_problem {
+  uint32_t grid_side;  /* n grid points per side, 1..8192 */
+  uint32_t snapshots;  /* S, 1..32, at t0 + s dt */
+  uint32_t wave_count; /* W, 1..128 */
+  uint32_t reserved;
+  double spacing;      /* metres between grid points */
+  double x0, z0;       /* world position of grid point (0, 0); point (i, j) = (x0 + i spacing, z0 + j spacing) */
+  double t0, dt;
+  double gravity;
+  const float* target; /* device [S][n*n][3] displacement (dx, y, dz), row-major points */
+  float* params;       /* device [W][5] {A, L, alpha, Q, phi0}; Adam updates it in place */
+} wt_fit_problem;
+
+typedef struct wt_adam_config {
+  float lr;
+  float beta1;
+  float beta2;
+  float eps;
+} wt_adam_config;
+
+/* A batch of fitting problems of any sizes; every call below covers the whole batch
+ * in one pair of kernel launches (adjoint, then reduce + Adam). Allocates the Adam
+ * moments and per-block partial sums once. */
+wt_status wt_fit_create(wt_ctx* ctx, const wt_fit_problem* problems, uint32_t count, wt_fit** out);
+void wt_fit_destroy(wt_fit* fit);
+/* Zeroes the Adam moments and step counters (a kernel, safe inside a captured graph). */
+wt_status wt_fit_reset(wt_fit* fit, void* stream);
+/* `iterations` Adam steps on every problem. The step counter lives on the device, so
+ * a captured sequence of iterations replays correctly. d_losses (may be NULL): device
+ * array of `count` floats, receives each problem's loss before its last step. */
+wt_status wt_fit_iterate(wt_fit* fit, const wt_adam_config* adam, uint32_t iterations, float* d_losses, void* stream);
+/* Loss and gradient at the current params: d_grad receives the problems' W*5
+ * gradients concatenated in batch order, d_losses (may be NULL) one loss each. */
+wt_status wt_fit_loss_grad(wt_fit* fit, float* d_grad, float* d_losses, void* stream);
+/* Writes the displacement of problem->params (taken as the hidden wave set) at every
+ * grid point and snapshot into d_target, plus N(0, sigma) noise per component from a
+ * counter-based generator keyed by seed (problem->target is ignored). */
+wt_status wt_fit_synthesize(wt_ctx* ctx, const wt_fit_problem* problem, float sigma, uint64_t seed, float* d_target,
+                            void* stream);
+/* Phase-storage policy of the adjoint: the bytes a stored (cos, sin) tensor of
+ * S x n^2 x W would take, and whether the kernels recompute the phases instead
+ * (always 1 in this build: recomputing by rotation is cheaper than the HBM round
+ * trip at every size). */
+wt_status wt_fit_phase_plan(const wt_fit_problem* problem, uint64_t budget_bytes, uint64_t* phase_tensor_bytes,
+                            int* recompute);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,349 @@
+// Adjoint of the heightfield fit and its Adam update (no reference counterpart).
+//
+// Model per wave (A, L, alpha, Q, phi0): w = 2 pi / L, omega = sqrt(g w),
+// D = (cos alpha, sin alpha), theta = w D.x + omega t + phi0 (the reference's sign
+// convention, assets/shaders/water-grid.vert:78), displacement
+// d = (Q A Dx cos theta, A sin theta, Q A Dz cos theta) (water-grid.vert:79-81).
+// Loss = 1 / (2 S V) * sum over snapshots and grid points of |d - target|^2.
+//
+// adjoint_kernel: one thread per grid point, all S snapshots in registers.
+//   forward: per wave, seed (cos, sin) from the FP64-reduced phase at t0 and step the
+//     snapshots with a rotation by omega dt; accumulate d per snapshot.
+//   residual r = d - target, loss contribution 1/2 |r|^2.
+//   backward: per wave, re-seed and step again (recomputing the phases is cheaper than
+//     storing them: 4 VALU ops per snapshot against an 8-byte HBM round trip), with
+//     nine running sums over snapshots, from which the five parameter derivatives of
+//     this grid point follow; then a fixed-order wavefront reduction (xor shuffles),
+//     a fixed-order sum over the 4 wavefronts, and one partial per (parameter, block)
+//     in global memory.
+// reduce_adam_kernel: one workgroup per (problem, parameter row): fixed-order sum of
+//   the block partials, scaling to the mean misfit, and either an Adam step on the
+//   parameter or a plain gradient / loss write-out.
+// Both kernels walk a table of problems, so one launch covers a whole batch; a single
+// problem is a table of one.
+#include <hip/hip_runtime.h>
+
+#include "wt_fit.hpp"
+
+namespace wt {
+
+constexpr int kFitThreads = 256;
+
+struct FitWaveLds {
+  double kx, kz, c0;   // revolutions per metre along x / z, phase at t0 (fract)
+  double om_rev;       // omega / 2 pi, revolutions per second
+  float qadx, qadz, A, Q, Dx, Dz;
+  float rc, rs;        // rotation by omega dt
+  float w, wL, om2L;   // w, w / L, omega / (2 L)
+};
+
+__device__ __forceinline__ int find_problem(const FitDesc* __restrict__ probs, int count, long long key,
+                                            bool by_param) {
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    const long long start = by_param ? probs[mid].row_offset : probs[mid].block_offset;
+    if (start <= key) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+
+__device__ __forceinline__ void stage_fit_wave(const FitDesc& P, int i, FitWaveLds* o) {
+  const float* pr = P.params + 5 * i;
+  const double A = pr[0], L = pr[1] > 1e-3f ? pr[1] : 1e-3f, al = pr[2], Q = pr[3], ph0 = pr[4];
+  const double w = kTwoPi / L, om = sqrt(P.g * w);
+  double sa, ca;
+  sincos(al, &sa, &ca);
+  o->kx = w * ca * kInvTwoPi;
+  o->kz = w * sa * kInvTwoPi;
+  const double ph = (om * P.t0 + ph0) * kInvTwoPi;
+  o->c0 = ph - floor(ph);
+  o->om_rev = om * kInvTwoPi;
+  const double step = om * P.dt * kInvTwoPi;
+  const double ang = (step - floor(step)) * kTwoPi;
+  double sn, cs;
+  sincos(ang, &sn, &cs);
+  o->rc = (float)cs;
+  o->rs = (float)sn;
+  o->A = (float)A;
+  o->Q = (float)Q;
+  o->Dx = (float)ca;
+  o->Dz = (float)sa;
+  o->qadx = (float)(Q * A * ca);
+  o->qadz = (float)(Q * A * sa);
+  o->w = (float)w;
+  o->wL = (float)(w / L);
+  o->om2L = (float)(om / (2.0 * L));
+}
+
+template <int SMAX>
+__device__ __forceinline__ void adjoint_body(const FitDesc& P, int local_block, const FitWaveLds* s_wave,
+                                             float* s_red) {
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int V = P.n * P.n;
+  const int v = local_block * kFitThreads + tid;
+  const bool active = v < V;
+  const int vv = active ? v : 0;
+  const double x = P.x0 + (double)(vv % P.n) * P.spacing;
+  const double z = P.z0 + (double)(vv / P.n) * P.spacing;
+  const int S = P.S;
+
+  float r[SMAX][3];
+#pragma unroll
+  for (int s = 0; s < SMAX; ++s) r[s][0] = r[s][1] = r[s][2] = 0.0f;
+
+  // forward: d per snapshot
+  for (int i = 0; i < P.W; ++i) {
+    const FitWaveLds& wv = s_wave[i];
+    const float ph = (float)__builtin_amdgcn_fract(fma(wv.kx, x, fma(wv.kz, z, wv.c0)));
+    float c = __builtin_amdgcn_cosf(ph), sn = __builtin_amdgcn_sinf(ph);
+    const float qadx = wv.qadx, qadz = wv.qadz, A = wv.A, rc = wv.rc, rs = wv.rs, nrs = -wv.rs;
+#pragma unroll
+    for (int s = 0; s < SMAX; ++s) {
+      if (s < S) {
+        r[s][0] = fmaf(qadx, c, r[s][0]);
+        r[s][1] = fmaf(A, sn, r[s][1]);
+        r[s][2] = fmaf(qadz, c, r[s][2]);
+        const float cn = fmaf(sn, nrs, c * rc);
+        sn = fmaf(c, rs, sn * rc);
+        c = cn;
+      }
+    }
+  }
+  // residuals and loss
+  float loss = 0.0f;
+#pragma unroll
+  for (int s = 0; s < SMAX; ++s) {
+    if (s < S) {
+      const float* tg = P.target + ((size_t)s * V + vv) * 3;
+      const float t0 = tg[0], t1 = tg[1], t2 = tg[2];
+      r[s][0] = active ? r[s][0] - t0 : 0.0f;
+      r[s][1] = active ? r[s][1] - t1 : 0.0f;
+      r[s][2] = active ? r[s][2] - t2 : 0.0f;
+      loss = fmaf(0.5f * r[s][0], r[s][0], loss);
+      loss = fmaf(0.5f * r[s][1], r[s][1], loss);
+      loss = fmaf(0.5f * r[s][2], r[s][2], loss);
+    }
+  }
+  const float lsum = wave_sum(loss);
+  if (lane == 0) s_red[(P.W * 5) * 4 + wid] = lsum;
+
+  // backward: nine sums over snapshots per wave, then the five derivatives
+  const float xf = (float)(x - P.x0), zf = (float)(z - P.z0);  // local coordinates for the D.x terms
+  const float x0f = (float)P.x0, z0f = (float)P.z0;
+  for (int i = 0; i < P.W; ++i) {
+    const FitWaveLds& wv = s_wave[i];
+    const float ph = (float)__builtin_amdgcn_fract(fma(wv.kx, x, fma(wv.kz, z, wv.c0)));
+    float c = __builtin_amdgcn_cosf(ph), sn = __builtin_amdgcn_sinf(ph);
+    const float rc = wv.rc, rs = wv.rs, nrs = -wv.rs;
+    float sxc = 0, szc = 0, syc = 0, sxs = 0, szs = 0, sys = 0, tsyc = 0, tsxs = 0, tszs = 0;
+    float t = 0.0f;  // time since t0 of snapshot s
+    const float dt = (float)P.dt;
+#pragma unroll
+    for (int s = 0; s < SMAX; ++s) {
+      if (s < S) {
+        const float rx = r[s][0], ry = r[s][1], rz = r[s][2];
+        sxc = fmaf(rx, c, sxc);
+        szc = fmaf(rz, c, szc);
+        syc = fmaf(ry, c, syc);
+        sxs = fmaf(rx, sn, sxs);
+        szs = fmaf(rz, sn, szs);
+        sys = fmaf(ry, sn, sys);
+        const float tc = t * c, ts = t * sn;
+        tsyc = fmaf(ry, tc, tsyc);
+        tsxs = fmaf(rx, ts, tsxs);
+        tszs = fmaf(rz, ts, tszs);
+        const float cn = fmaf(sn, nrs, c * rc);
+        sn = fmaf(c, rs, sn * rc);
+        c = cn;
+        t += dt;
+      }
+    }
+    const float A = wv.A, Q = wv.Q, Dx = wv.Dx, Dz = wv.Dz;
+    const float sec = Dx * sxc + Dz * szc;      // sum of (r . D) cos
+    const float sepc = -Dz * sxc + Dx * szc;    // sum of (r . D-perp) cos
+    const float g0 = A * (syc - Q * (Dx * sxs + Dz * szs));          // sum of r . dd/dtheta
+    const float gt = A * (tsyc - Q * (Dx * tsxs + Dz * tszs));       // sum of (t - t0) r . dd/dtheta
+    const float g0t = fmaf((float)P.t0, g0, gt);                     // sum of t r . dd/dtheta
+    // D.x and D-perp.x split into the problem origin term and the local term
+    const float dotx = fmaf(Dx, xf, Dz * zf) + fmaf(Dx, x0f, Dz * z0f);
+    const float perpx = fmaf(-Dz, xf, Dx * zf) + fmaf(-Dz, x0f, Dx * z0f);
+    float gA = fmaf(Q, sec, sys);
+    float gL = -wv.wL * dotx * g0 - wv.om2L * g0t;
+    float gAl = fmaf(wv.w * perpx, g0, Q * A * sepc);
+    float gQ = A * sec;
+    float gP = g0;
+    if (!active) gA = gL = gAl = gQ = gP = 0.0f;
+    gA = wave_sum(gA);
+    gL = wave_sum(gL);
+    gAl = wave_sum(gAl);
+    gQ = wave_sum(gQ);
+    gP = wave_sum(gP);
+    if (lane == 0) {
+      float* o = s_red + (i * 5) * 4 + wid;
+      o[0 * 4] = gA;
+      o[1 * 4] = gL;
+      o[2 * 4] = gAl;
+      o[3 * 4] = gQ;
+      o[4 * 4] = gP;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kFitThreads) void adjoint_kernel(const FitDesc* __restrict__ probs, int count,
+                                                              int bump_step) {
+  __shared__ FitWaveLds s_wave[kFitMaxWaves];
+  __shared__ float s_red[(kFitMaxWaves * 5 + 1) * 4];
+  __shared__ int s_prob;
+  if (threadIdx.x == 0) s_prob = find_problem(probs, count, blockIdx.x, false);
+  __syncthreads();
+  const FitDesc P = probs[s_prob];
+  const int local_block = (int)(blockIdx.x - P.block_offset);
+  if (bump_step && local_block == 0 && threadIdx.x == 0) P.step[0] += 1;  // Adam step of this iteration
+  for (int i = threadIdx.x; i < P.W; i += blockDim.x) stage_fit_wave(P, i, &s_wave[i]);
+  __syncthreads();
+  if (P.S <= 8) adjoint_body<8>(P, local_block, s_wave, s_red);
+  else if (P.S <= 16) adjoint_body<16>(P, local_block, s_wave, s_red);
+  else adjoint_body<32>(P, local_block, s_wave, s_red);
+  __syncthreads();
+  // fixed-order sum over the 4 wavefronts, one partial per (row, block)
+  const int rows = P.W * 5 + 1;
+  for (int j = threadIdx.x; j < rows; j += blockDim.x) {
+    const float* q = s_red + j * 4;
+    P.partials[(size_t)j * P.nblocks + local_block] = ((q[0] + q[1]) + q[2]) + q[3];
+  }
+}
+
+__global__ __launch_bounds__(kFitThreads) void reduce_adam_kernel(const FitDesc* __restrict__ probs, int count,
+                                                                  FitAdam adam, int mode, float* __restrict__ grad_out,
+                                                                  float* __restrict__ loss_out) {
+  __shared__ float s[kFitThreads];
+  __shared__ int s_prob;
+  if (threadIdx.x == 0) s_prob = find_problem(probs, count, blockIdx.x, true);
+  __syncthreads();
+  const FitDesc P = probs[s_prob];
+  const int row = (int)(blockIdx.x - P.row_offset);
+  const float* src = P.partials + (size_t)row * P.nblocks;
+  float acc = 0.0f;
+  for (int b = threadIdx.x; b < P.nblocks; b += blockDim.x) acc += src[b];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int m = kFitThreads / 2; m >= 1; m >>= 1) {
+    if (threadIdx.x < m) s[threadIdx.x] += s[threadIdx.x + m];
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  const double inv = 1.0 / ((double)P.S * (double)P.n * (double)P.n);
+  const float total = (float)((double)s[0] * inv);
+  const int nparam = P.W * 5;
+  if (row == nparam) {  // loss row
+    if (loss_out) loss_out[s_prob] = total;
+    P.loss[0] = total;
+    return;
+  }
+  if (mode == 0) {  // gradient only
+    grad_out[P.param_out_offset + row] = total;
+    return;
+  }
+  // Adam (Kingma & Ba, 2015) with bias correction; step counter kept on the device
+  const int t = P.step[0];
+  const float g = total;
+  float m = adam.beta1 * P.adam_m[row] + (1.0f - adam.beta1) * g;
+  float v2 = adam.beta2 * P.adam_v[row] + (1.0f - adam.beta2) * g * g;
+  P.adam_m[row] = m;
+  P.adam_v[row] = v2;
+  const float bc1 = 1.0f - powf(adam.beta1, (float)t), bc2 = 1.0f - powf(adam.beta2, (float)t);
+  const float mhat = m / bc1, vhat = v2 / bc2;
+  float p = P.params[row] - adam.lr * mhat / (sqrtf(vhat) + adam.eps);
+  if (row % 5 == 1 && p < 1e-3f) p = 1e-3f;  // wavelength stays positive
+  P.params[row] = p;
+}
+
+// Synthetic target: displacement of the problem's wave set (read from params) at every
+// grid point and snapshot, plus N(0, sigma) noise per component from a counter-based
+// generator (SplitMix64 of seed and element index, Box-Muller), so the target does not
+// depend on the launch shape. One thread per grid point, phases as in the adjoint.
+__device__ __forceinline__ unsigned long long splitmix64(unsigned long long z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ double unit_double(unsigned long long z) { return (double)(z >> 11) * (1.0 / 9007199254740992.0); }
+
+__global__ __launch_bounds__(kFitThreads) void synthesize_kernel(FitDesc P, float* __restrict__ out, float sigma,
+                                                                 unsigned long long seed) {
+  __shared__ FitWaveLds s_wave[kFitMaxWaves];
+  for (int i = threadIdx.x; i < P.W; i += blockDim.x) stage_fit_wave(P, i, &s_wave[i]);
+  __syncthreads();
+  const long long V = (long long)P.n * P.n;
+  const long long v = (long long)blockIdx.x * kFitThreads + threadIdx.x;
+  if (v >= V) return;
+  const double x = P.x0 + (double)(v % P.n) * P.spacing;
+  const double z = P.z0 + (double)(v / P.n) * P.spacing;
+  for (int s = 0; s < P.S; ++s) {
+    float d0 = 0.0f, d1 = 0.0f, d2 = 0.0f;
+    for (int i = 0; i < P.W; ++i) {
+      const FitWaveLds& wv = s_wave[i];
+      // phase at snapshot s in FP64: phase at t0 plus omega s dt, reduced to one revolution
+      const double ph = fma(wv.kx, x, fma(wv.kz, z, wv.c0)) + wv.om_rev * (P.dt * s);
+      const float f = (float)__builtin_amdgcn_fract(ph);
+      const float c = __builtin_amdgcn_cosf(f), sn = __builtin_amdgcn_sinf(f);
+      d0 = fmaf(wv.qadx, c, d0);
+      d1 = fmaf(wv.A, sn, d1);
+      d2 = fmaf(wv.qadz, c, d2);
+    }
+    float* o = out + ((size_t)s * V + v) * 3;
+    const float d[3] = {d0, d1, d2};
+    for (int k = 0; k < 3; ++k) {
+      float nz = 0.0f;
+      if (sigma > 0.0f) {
+        const unsigned long long idx = ((unsigned long long)s * V + v) * 3 + k;
+        const double u1 = fmax(unit_double(splitmix64(seed ^ (2 * idx))), 1e-300);
+        const double u2 = unit_double(splitmix64(seed ^ (2 * idx + 1)));
+        nz = (float)(sqrt(-2.0 * log(u1)) * cos(kTwoPi * u2));
+      }
+      o[k] = fmaf(sigma, nz, d[k]);
+    }
+  }
+}
+
+hipError_t launch_synthesize(const FitDesc& P, float* out, float sigma, unsigned long long seed, hipStream_t st) {
+  const long long V = (long long)P.n * P.n;
+  const long long blocks = (V + kFitThreads - 1) / kFitThreads;
+  hipLaunchKernelGGL(synthesize_kernel, dim3((unsigned)blocks), dim3(kFitThreads), 0, st, P, out, sigma, seed);
+  return hipGetLastError();
+}
+
+__global__ void zero_words_kernel(uint32_t* __restrict__ p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+hipError_t launch_zero_words(uint32_t* p, size_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(zero_words_kernel, dim3((unsigned)blocks), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_fit_iteration(const FitDesc* d_probs, int count, long long total_blocks, long long total_rows,
+                                const FitAdam& adam, int mode, float* grad_out, float* loss_out, hipStream_t st) {
+  hipLaunchKernelGGL(adjoint_kernel, dim3((unsigned)total_blocks), dim3(kFitThreads), 0, st, d_probs, count,
+                     mode == 1 ? 1 : 0);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(reduce_adam_kernel, dim3((unsigned)total_rows), dim3(kFitThreads), 0, st, d_probs, count, adam,
+                     mode, grad_out, loss_out);
+  return hipGetLastError();
+}
+
+}  // namespace wt

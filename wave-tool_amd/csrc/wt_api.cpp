@@ -98,6 +98,8 @@ double reduce_offset(double v, int wrap) {
 }  // namespace
 
 namespace wt {
+wt_status ctx_fail(wt_ctx* ctx, wt_status s, const std::string& msg) { return fail(ctx, s, msg); }
+
 // shared with the batched / adjoint translation units
 wt_status validate_heightmap(wt_ctx* ctx, const wt_heightmap* hm, bool check_ptr) {
   if (!hm) return WT_OK;

@@ -1,0 +1,46 @@
+// Device descriptors of the heightfield-fit problems and the launcher shared by the
+// C ABI and the adjoint kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "wt_device.hpp"
+
+namespace wt {
+
+constexpr int kFitMaxWaves = 128;
+constexpr int kFitMaxSnapshots = 32;
+
+// One problem of a batch as the kernels see it. Device pointers are owned by the
+// wt_fit handle except params and target, which the caller owns.
+struct FitDesc {
+  double spacing, x0, z0, t0, dt, g;
+  const float* target;  // [S][n*n][3]
+  float* params;        // [W][5] {A, L, alpha, Q, phi0}
+  float* adam_m;        // [W*5]
+  float* adam_v;        // [W*5]
+  float* partials;      // [W*5 + 1][nblocks]: per-block sums of r . dd/dp and of |r|^2 / 2
+  float* loss;          // [1] loss of the last evaluation
+  int* step;            // [1] Adam step counter
+  int n, S, W, nblocks;
+  long long block_offset;      // first block of this problem in the adjoint grid
+  long long row_offset;        // first row of this problem in the reduce grid
+  long long param_out_offset;  // first parameter of this problem in a concatenated gradient
+};
+
+struct FitAdam {
+  float lr, beta1, beta2, eps;
+};
+
+// mode 0: gradient + loss only (grad_out receives the concatenated gradients);
+// mode 1: Adam step (the adjoint launch bumps each problem's step counter first).
+hipError_t launch_fit_iteration(const FitDesc* d_probs, int count, long long total_blocks, long long total_rows,
+                                const FitAdam& adam, int mode, float* grad_out, float* loss_out, hipStream_t st);
+
+// Target of a problem from its own params (the hidden wave set) plus N(0, sigma) noise.
+hipError_t launch_synthesize(const FitDesc& P, float* out, float sigma, unsigned long long seed, hipStream_t st);
+
+// Zero n 32-bit words with a kernel (graph-capture safe).
+hipError_t launch_zero_words(uint32_t* p, size_t n, hipStream_t st);
+
+}  // namespace wt

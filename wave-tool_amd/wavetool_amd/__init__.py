@@ -8,7 +8,8 @@ kernels in libwavetool.so.
 import ctypes
 
 from . import scenes
-from ._abi import (LaunchConfig, Camera, GerstnerWave, Heightmap, VertexBuffers, WaterParams, WtError, check, lib,
+from ._abi import (LaunchConfig, Camera, GerstnerWave, Heightmap, VertexBuffers, WaterParams, FitProblem, AdamConfig,
+                   WtError, check, lib,
                    WT_OK, WT_ERR_INVALID_ARG, WT_ERR_NO_INTERSECTION, WT_ERR_DEGENERATE, WT_WRAP_REPEAT,
                    WT_WRAP_MIRRORED_REPEAT, WT_TEXEL_R8_UNORM, WT_TEXEL_R32F, WT_FLOP_PER_VERTEX_WAVE,
                    WT_BYTES_PER_VERTEX_PN, WT_BYTES_PER_VERTEX_PNTB)
@@ -143,6 +144,63 @@ class Context:
         g = ctypes.c_void_p()
         self._check(lib().wt_graph_end(self.handle, ctypes.c_void_p(stream.cuda_stream), ctypes.byref(g)))
         return Graph(g)
+
+
+def fit_problem(n, S, W, spacing, x0, z0, t0, dt, target, params, gravity=9.81):
+    """FitProblem over device tensors target (S, n*n, 3) and params (W, 5), float32."""
+    p = FitProblem()
+    p.grid_side, p.snapshots, p.wave_count = int(n), int(S), int(W)
+    p.spacing, p.x0, p.z0, p.t0, p.dt, p.gravity = (float(v) for v in (spacing, x0, z0, t0, dt, gravity))
+    p.target = None if target is None else target.data_ptr()
+    p.params = params.data_ptr()
+    return p
+
+
+def synthesize(ctx, problem, sigma, seed, target, stream=None):
+    """Fill the device tensor `target` with the displacement of problem.params plus noise."""
+    ctx._check(lib().wt_fit_synthesize(ctx.handle, ctypes.byref(problem), float(sigma), int(seed),
+                                       ctypes.c_void_p(target.data_ptr()), _stream(stream)))
+
+
+def phase_plan(problem, budget_bytes=0):
+    b = ctypes.c_uint64()
+    r = ctypes.c_int()
+    check(lib().wt_fit_phase_plan(ctypes.byref(problem), int(budget_bytes), ctypes.byref(b), ctypes.byref(r)))
+    return b.value, bool(r.value)
+
+
+class Fit:
+    """A batch of heightfield-fitting problems (wt_fit)."""
+
+    def __init__(self, ctx, problems):
+        self.ctx = ctx
+        self.count = len(problems)
+        arr = (FitProblem * self.count)(*problems)
+        h = ctypes.c_void_p()
+        ctx._check(lib().wt_fit_create(ctx.handle, arr, self.count, ctypes.byref(h)))
+        self.handle = h
+
+    def reset(self, stream=None):
+        self.ctx._check(lib().wt_fit_reset(self.handle, _stream(stream)))
+
+    def iterate(self, iterations, lr=1e-2, beta1=0.9, beta2=0.999, eps=1e-8, losses=None, stream=None):
+        cfg = AdamConfig(lr, beta1, beta2, eps)
+        self.ctx._check(lib().wt_fit_iterate(self.handle, ctypes.byref(cfg), int(iterations), _ptr(losses),
+                                             _stream(stream)))
+
+    def loss_grad(self, grad, losses=None, stream=None):
+        self.ctx._check(lib().wt_fit_loss_grad(self.handle, _ptr(grad), _ptr(losses), _stream(stream)))
+
+    def close(self):
+        if self.handle:
+            lib().wt_fit_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Graph:

@@ -95,6 +95,28 @@ class LaunchConfig(ctypes.Structure):
     ]
 
 
+class FitProblem(ctypes.Structure):
+    _fields_ = [
+        ("grid_side", ctypes.c_uint32),
+        ("snapshots", ctypes.c_uint32),
+        ("wave_count", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("spacing", ctypes.c_double),
+        ("x0", ctypes.c_double),
+        ("z0", ctypes.c_double),
+        ("t0", ctypes.c_double),
+        ("dt", ctypes.c_double),
+        ("gravity", ctypes.c_double),
+        ("target", ctypes.c_void_p),
+        ("params", ctypes.c_void_p),
+    ]
+
+
+class AdamConfig(ctypes.Structure):
+    _fields_ = [("lr", ctypes.c_float), ("beta1", ctypes.c_float), ("beta2", ctypes.c_float),
+                ("eps", ctypes.c_float)]
+
+
 # (name, restype, argtypes) for every entry point declared in include/wt.h
 _vp = ctypes.c_void_p
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -122,6 +144,15 @@ PROTOTYPES = [
     ("wt_graph_end", ctypes.c_int, [_vp, _vp, ctypes.POINTER(_vp)]),
     ("wt_graph_launch", ctypes.c_int, [_vp, _vp]),
     ("wt_graph_destroy", None, [_vp]),
+    ("wt_fit_create", ctypes.c_int, [_vp, ctypes.POINTER(FitProblem), ctypes.c_uint32, ctypes.POINTER(_vp)]),
+    ("wt_fit_destroy", None, [_vp]),
+    ("wt_fit_reset", ctypes.c_int, [_vp, _vp]),
+    ("wt_fit_iterate", ctypes.c_int, [_vp, ctypes.POINTER(AdamConfig), ctypes.c_uint32, _vp, _vp]),
+    ("wt_fit_loss_grad", ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    ("wt_fit_synthesize", ctypes.c_int,
+     [_vp, ctypes.POINTER(FitProblem), ctypes.c_float, ctypes.c_uint64, _vp, _vp]),
+    ("wt_fit_phase_plan", ctypes.c_int,
+     [ctypes.POINTER(FitProblem), ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int)]),
 ]
 
 _lib = None
