@@ -201,6 +201,40 @@ wt_status wt_autotune_forward(wt_ctx* ctx, uint32_t grid_length, uint32_t wave_c
 wt_status wt_autotune_save(wt_ctx* ctx, const char* path);
 wt_status wt_autotune_load(wt_ctx* ctx, const char* path);
 
+/* ---------------------------------------------------------------- batched forward */
+/* One ocean patch of a batch: its own projected grid (corners from
+ * wt_projector_corners of its own camera), time, scales and a run of the batch's
+ * wave array. Outputs go to [vertex_offset, vertex_offset + gridLength^2) of the
+ * batch's output arrays. The reference draws one grid per frame
+ * (src/render-engine.cpp:1115); a batch replaces many such draws. */
+typedef struct wt_patch {
+  double corners[4][4];
+  float cameraPosition[3];
+  uint32_t gridLength;
+  uint32_t wave_offset;
+  uint32_t wave_count;
+  uint64_t vertex_offset;
+  double waveAnimationTimeInSeconds;
+  float heightmapSampleScale;
+  float heightmapDisplacementScale;
+  float verticalBounceWaveDisplacement;
+  float heightmapScrollVelocity[2];
+  float reserved;
+} wt_patch;
+
+/* Validates the patches, uploads them, and builds the work list (units of one
+ * workgroup's vertices of one patch, most expensive first) for a persistent launch. */
+wt_status wt_batch_create(wt_ctx* ctx, const wt_patch* patches, uint32_t count, wt_batch** out);
+void wt_batch_destroy(wt_batch* batch);
+/* Total vertices of the batch (the size the output arrays need, in float3). */
+wt_status wt_batch_vertex_count(const wt_batch* batch, uint64_t* vertices);
+/* One persistent launch over every patch: each patch's vertex program at
+ * time + time_offset. d_waves holds total_waves waves that the patches index into.
+ * Replayable from a captured graph (the work counter resets itself). */
+wt_status wt_water_displace_batched(wt_ctx* ctx, wt_batch* batch, const wt_gerstner_wave* d_waves, uint32_t total_waves,
+                                    const wt_heightmap* heightmap, const wt_vertex_buffers* out, double time_offset,
+                                    void* stream);
+
 /* ---------------------------------------------------------------- graphs */
 /* Stream capture of any sequence of wt_* calls into a HIP graph, so a frame
  * loop replays with one launch (the reference's per-frame render loop,

@@ -1,0 +1,143 @@
+"""GPU tests of the persistent batched forward kernel (wt_water_displace_batched)."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as orc
+import wavetool_amd as wt
+from wavetool_amd import scenes
+from helpers import POS_TOL, NRM_TOL, compare, near_texel_edge
+
+pytestmark = pytest.mark.gpu
+NTHREADS = min(16, os.cpu_count() or 1)
+
+
+def say(msg):
+    print("[case] " + msg, flush=True)
+
+
+def run_batch(torch, ctx, specs, texels, time_offset=0.0):
+    dev = torch.device("cuda")
+    patches, waves, V = wt.batch_inputs(specs)
+    batch = wt.Batch(ctx, patches)
+    assert batch.vertices == V
+    w = torch.from_numpy(waves).to(dev)
+    hm = torch.from_numpy(texels).to(dev)
+    pos = torch.full((V * 3,), float("nan"), device=dev)
+    nrm = torch.full((V * 3,), float("nan"), device=dev)
+    batch.displace(w, pos, nrm, heightmap=hm, time_offset=time_offset)
+    torch.cuda.synchronize()
+    return batch, patches, waves, w, hm, pos, nrm
+
+
+def batch_shape(specs):
+    return (2, 8, 64) if max(s["N"] for s in specs) > 256 else (1, 4, 32)
+
+
+@pytest.mark.parametrize("count,seed", [(24, 5), (7, 6)])
+def test_batched_equals_plain_per_patch(torch_cuda, wtctx, count, seed):
+    """Every patch of a batch equals the plain kernel run on that patch alone with the
+    same launch shape, bit for bit; so the batch inherits the plain kernel's parity."""
+    torch = torch_cuda
+    specs = scenes.batched_patches(seed=seed, count=count)
+    if seed == 6:  # a batch of small grids only (the narrow shape)
+        for s in specs:
+            s["N"] = max(2, s["N"] // 8)
+    say("batched vs plain: %d patches, sides %s" % (count, [s["N"] for s in specs]))
+    texels = scenes.value_noise(3, 128)
+    batch, patches, waves, w, hm, pos, nrm = run_batch(torch, wtctx, specs, texels, time_offset=0.5)
+    wtctx.set_launch_config(*batch_shape(specs), 64)
+    try:
+        for i, (s, p) in enumerate(zip(specs, patches)):
+            N = s["N"]
+            params = wt.water_params(np.ctypeslib.as_array(p.corners).copy(), s["camera"][4:7], N, s["time"] + 0.5, 0.02, 0.05, 0.0,
+                                     (0.3, 0.0))
+            P = torch.empty(N * N * 3, device="cuda")
+            Nn = torch.empty_like(P)
+            wtctx.displace(params, w[p.wave_offset:p.wave_offset + p.wave_count], P, Nn, heightmap=hm,
+                           wave_count=p.wave_count)
+            torch.cuda.synchronize()
+            o = p.vertex_offset * 3
+            assert torch.equal(P, pos[o:o + N * N * 3]), i
+            assert torch.equal(Nn, nrm[o:o + N * N * 3]), i
+    finally:
+        wtctx.set_launch_config()
+    batch.close()
+
+
+def test_batched_oracle_parity_long_times(torch_cuda, wtctx):
+    """Patches at t up to 3600 s against the FP64 oracle (FP64 phase reduction)."""
+    torch = torch_cuda
+    specs = scenes.batched_patches(seed=9, count=6)
+    for s in specs:
+        s["N"] = min(s["N"], 200)
+    say("batched oracle parity, times %s" % [round(s["time"], 1) for s in specs])
+    texels = scenes.value_noise(4, 64)
+    batch, patches, waves, w, hm, pos, nrm = run_batch(torch, wtctx, specs, texels)
+    P = pos.cpu().numpy().reshape(-1, 3)
+    Nn = nrm.cpu().numpy().reshape(-1, 3)
+    for s, p in zip(specs, patches):
+        N = s["N"]
+        sd = orc.SceneData(np.ctypeslib.as_array(p.corners).copy(), s["camera"][4:7], N, waves[p.wave_offset:p.wave_offset + p.wave_count],
+                           s["time"], 0.02, 0.05, 0.0, (0.3, 0.0), texels)
+        ref = orc.vertices(sd, nthreads=NTHREADS, want=("pos", "nrm"))
+        o = p.vertex_offset
+        out = dict(pos=P[o:o + N * N], nrm=Nn[o:o + N * N])
+        edge = near_texel_edge(ref["pos"], 0.02, (0.3, 0.0), s["time"], 64, 64)
+        res = compare(out, ref, s["camera"][4:7], edge)
+        print(N, s["W"], res, flush=True)
+        assert res["pos_max"] <= POS_TOL and res["nrm_max"] <= NRM_TOL
+    batch.close()
+
+
+def test_batched_graph_replay(torch_cuda, wtctx):
+    """Eight frames of the batch captured in one graph; replays equal eager frames (the
+    work counter resets itself at the end of every launch)."""
+    torch = torch_cuda
+    say("batched graph replay")
+    specs = scenes.batched_patches(seed=11, count=12)
+    texels = scenes.value_noise(3, 128)
+    batch, patches, waves, w, hm, pos, nrm = run_batch(torch, wtctx, specs, texels)
+    F = 8
+    V = batch.vertices
+    outp = torch.zeros((F, V * 3), device="cuda")
+    outn = torch.zeros_like(outp)
+    s = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    for f in range(F):
+        batch.displace(w, outp[f], outn[f], heightmap=hm, time_offset=f / 60.0, stream=s)
+    s.synchronize()
+    ep, en = outp.clone(), outn.clone()
+    wtctx.graph_begin(s)
+    for f in range(F):
+        batch.displace(w, outp[f], outn[f], heightmap=hm, time_offset=f / 60.0, stream=s)
+    g = wtctx.graph_end(s)
+    for rep in range(3):
+        outp.zero_()
+        outn.zero_()
+        torch.cuda.synchronize()
+        g.launch(s)
+        s.synchronize()
+        assert torch.equal(outp, ep) and torch.equal(outn, en), rep
+    g.close()
+    batch.close()
+
+
+def test_batch_rejects_bad_input(torch_cuda, wtctx):
+    say("batch invalid input")
+    specs = scenes.batched_patches(seed=2, count=3)
+    patches, waves, V = wt.batch_inputs(specs)
+    bad = list(patches)
+    bad[1].gridLength = 1
+    with pytest.raises(wt.WtError) as e:
+        wt.Batch(wtctx, bad)
+    assert e.value.status == wt.WT_ERR_INVALID_ARG
+    patches, waves, V = wt.batch_inputs(specs)
+    b = wt.Batch(wtctx, patches)
+    short = torch_cuda.from_numpy(waves[:2]).cuda()
+    pos = torch_cuda.zeros(V * 3, device="cuda")
+    with pytest.raises(wt.WtError) as e:
+        b.displace(short, pos, pos.clone())
+    assert e.value.status == wt.WT_ERR_INVALID_ARG
+    b.close()

@@ -3,6 +3,7 @@
 // include/wt.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -437,6 +438,159 @@ wt_status wt_autotune_load(wt_ctx* ctx, const char* path) {
   }
   std::fclose(f);
   ctx->table = t;
+  return WT_OK;
+}
+
+}  // extern "C"
+
+struct wt_batch {
+  wt_ctx* ctx = nullptr;
+  std::vector<wt::PatchDesc> patches;
+  wt::PatchDesc* d_patches = nullptr;
+  wt::BatchUnit* d_units = nullptr;
+  unsigned* d_counters = nullptr;
+  int n_units = 0;
+  uint64_t vertices = 0;
+  uint32_t max_wave_end = 0;
+  wt::LaunchShape shape{};
+};
+
+extern "C" {
+
+wt_status wt_batch_create(wt_ctx* ctx, const wt_patch* patches, uint32_t count, wt_batch** out) {
+  if (!ctx || !out) return WT_ERR_INVALID_ARG;
+  *out = nullptr;
+  if (!patches || count == 0) return fail(ctx, WT_ERR_INVALID_ARG, "need at least one patch");
+  wt_batch* b = new (std::nothrow) wt_batch();
+  if (!b) return WT_ERR_OOM;
+  b->ctx = ctx;
+  uint32_t max_n = 0;
+  for (uint32_t i = 0; i < count; ++i) {
+    const wt_patch& p = patches[i];
+    const std::string tag = "patch " + std::to_string(i) + ": ";
+    bool ok = p.gridLength >= 2 && p.gridLength <= 32768 && p.wave_count <= WT_MAX_WAVES &&
+              std::isfinite(p.waveAnimationTimeInSeconds);
+    for (int r = 0; r < 4 && ok; ++r)
+      for (int k = 0; k < 4; ++k) ok = ok && std::isfinite(p.corners[r][k]);
+    const float fs[8] = {p.cameraPosition[0], p.cameraPosition[1], p.cameraPosition[2], p.heightmapSampleScale,
+                         p.heightmapDisplacementScale, p.verticalBounceWaveDisplacement,
+                         p.heightmapScrollVelocity[0], p.heightmapScrollVelocity[1]};
+    for (float f : fs) ok = ok && std::isfinite(f);
+    if (!ok) {
+      delete b;
+      return fail(ctx, WT_ERR_INVALID_ARG, tag + "gridLength in [2, 32768], wave_count <= WT_MAX_WAVES, finite values");
+    }
+    wt::PatchDesc d{};
+    const double(*c)[4] = p.corners;
+    d.bl_x = c[0][0]; d.bl_z = c[0][2];
+    d.du_x = c[2][0] - c[0][0]; d.du_z = c[2][2] - c[0][2];
+    d.dv_x = c[1][0] - c[0][0]; d.dv_z = c[1][2] - c[0][2];
+    d.duv_x = c[3][0] - c[1][0] - c[2][0] + c[0][0];
+    d.duv_z = c[3][2] - c[1][2] - c[2][2] + c[0][2];
+    d.inv_nm1 = 1.0 / (double)(p.gridLength - 1);
+    d.time = p.waveAnimationTimeInSeconds;
+    d.cam[0] = p.cameraPosition[0]; d.cam[1] = p.cameraPosition[1]; d.cam[2] = p.cameraPosition[2];
+    d.s_h = p.heightmapSampleScale;
+    d.s_d = p.heightmapDisplacementScale;
+    d.bounce = p.verticalBounceWaveDisplacement;
+    d.scroll_x = p.heightmapScrollVelocity[0];
+    d.scroll_z = p.heightmapScrollVelocity[1];
+    d.N = (int)p.gridLength;
+    d.W = (int)p.wave_count;
+    d.wave_offset = (int)p.wave_offset;
+    d.vertex_offset = (long long)p.vertex_offset;
+    b->patches.push_back(d);
+    b->vertices = std::max<uint64_t>(b->vertices, p.vertex_offset + (uint64_t)p.gridLength * p.gridLength);
+    b->max_wave_end = std::max<uint32_t>(b->max_wave_end, p.wave_offset + p.wave_count);
+    max_n = std::max(max_n, p.gridLength);
+  }
+  // one shape for the whole batch: the wide shape when any grid is wider than 256
+  b->shape = max_n > 256 ? wt::LaunchShape{2, 8, 64, 64} : wt::LaunchShape{1, 4, 32, 64};
+  std::vector<wt::BatchUnit> units;
+  for (uint32_t i = 0; i < count; ++i) {
+    const long long groups = wt::forward_segment_groups(b->patches[i].N, b->shape);
+    for (long long gidx = 0; gidx < groups; ++gidx) units.push_back(wt::BatchUnit{(int)i, (int)gidx});
+  }
+  // most expensive units first (a unit costs about its patch's wave count)
+  std::stable_sort(units.begin(), units.end(), [&](const wt::BatchUnit& x, const wt::BatchUnit& y) {
+    return b->patches[x.patch].W > b->patches[y.patch].W;
+  });
+  b->n_units = (int)units.size();
+  hipError_t e = hipMalloc(&b->d_patches, sizeof(wt::PatchDesc) * count);
+  if (e == hipSuccess) e = hipMalloc(&b->d_units, sizeof(wt::BatchUnit) * units.size());
+  if (e == hipSuccess) e = hipMalloc(&b->d_counters, 2 * sizeof(unsigned));
+  if (e == hipSuccess)
+    e = hipMemcpy(b->d_patches, b->patches.data(), sizeof(wt::PatchDesc) * count, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(b->d_units, units.data(), sizeof(wt::BatchUnit) * units.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(b->d_counters, 0, 2 * sizeof(unsigned));
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    wt_status s = hip_fail(ctx, e, "wt_batch_create");
+    wt_batch_destroy(b);
+    return s;
+  }
+  *out = b;
+  return WT_OK;
+}
+
+void wt_batch_destroy(wt_batch* b) {
+  if (!b) return;
+  if (b->d_patches) (void)hipFree(b->d_patches);
+  if (b->d_units) (void)hipFree(b->d_units);
+  if (b->d_counters) (void)hipFree(b->d_counters);
+  delete b;
+}
+
+wt_status wt_batch_vertex_count(const wt_batch* b, uint64_t* vertices) {
+  if (!b || !vertices) return WT_ERR_INVALID_ARG;
+  *vertices = b->vertices;
+  return WT_OK;
+}
+
+wt_status wt_water_displace_batched(wt_ctx* ctx, wt_batch* b, const wt_gerstner_wave* d_waves, uint32_t total_waves,
+                                    const wt_heightmap* hm, const wt_vertex_buffers* out, double time_offset,
+                                    void* stream) {
+  if (!ctx || !b || !out) return WT_ERR_INVALID_ARG;
+  if (total_waves < b->max_wave_end) return fail(ctx, WT_ERR_INVALID_ARG, "a patch indexes past total_waves");
+  if (total_waves && !d_waves) return fail(ctx, WT_ERR_INVALID_ARG, "d_waves is NULL");
+  if (!out->position || !out->normal) return fail(ctx, WT_ERR_INVALID_ARG, "position and normal buffers are required");
+  if (!std::isfinite(time_offset)) return fail(ctx, WT_ERR_INVALID_ARG, "time_offset must be finite");
+  hipStream_t st = (hipStream_t)stream;
+  const bool capturing = stream_capturing(st);
+  wt_status hs = wt::validate_heightmap(ctx, hm, !capturing);
+  if (hs != WT_OK) return hs;
+  const void* bufs[4] = {out->position, out->normal, out->tangent, out->bitangent};
+  for (const void* p : bufs)
+    if (p && (reinterpret_cast<uintptr_t>(p) & 3)) return fail(ctx, WT_ERR_INVALID_ARG, "output buffers must be 4-byte aligned");
+  if (!capturing) {
+    for (const void* p : bufs)
+      if (p && !is_device_pointer(p)) return fail(ctx, WT_ERR_INVALID_ARG, "output buffers must be device memory");
+    if (total_waves && !is_device_pointer(d_waves)) return fail(ctx, WT_ERR_INVALID_ARG, "d_waves must be device memory");
+  }
+  wt::BatchArgs a{};
+  a.patches = b->d_patches;
+  a.units = b->d_units;
+  a.n_units = b->n_units;
+  a.counters = b->d_counters;
+  a.time_offset = time_offset;
+  a.waves = d_waves;
+  if (hm) {
+    a.texels = hm->texels;
+    a.tex_w = (int)hm->width;
+    a.tex_h = (int)hm->height;
+    a.tex_fmt = (int)hm->format;
+    a.tex_wrap = (int)hm->wrap;
+  }
+  a.pos = out->position;
+  a.nrm = out->normal;
+  a.tan = out->tangent;
+  a.bit = out->bitangent;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || cus <= 0) cus = 256;
+  const long long resident = (long long)wt::batched_blocks_per_cu(b->shape, hm != nullptr) * cus;
+  const int grid = (int)std::min<long long>(resident, b->n_units);
+  hipError_t e = wt::launch_forward_batched(a, b->shape, grid, st);
+  if (e != hipSuccess) return hip_fail(ctx, e, "water_forward_batched_kernel launch");
   return WT_OK;
 }
 

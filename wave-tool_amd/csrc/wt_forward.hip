@@ -91,20 +91,31 @@ __device__ __forceinline__ void epilogue_steps(const FwdParams& p, const RowGeom
   }
 }
 
+// LDS of one workgroup of the forward kernels.
+template <int K, int LC, int S>
+struct SegShared {
+  static constexpr int LR = 64 / LC;
+  static constexpr int KS = K / S;
+  static constexpr int kSegPerWG = kWavesPerWG / S;
+  WaveLds wave[kChunk];
+  float4 rot[kWavesPerWG][LR * kChunk];  // (cos, sin, -sin, 0) per (row, wave)
+  // partial sums handed to the other wavefront of the pair: [pair][sender][K-KS][8][64]
+  float xchg[S > 1 ? kSegPerWG : 1][S > 1 ? S : 1][S > 1 ? (K - KS) * 8 * 64 : 1];
+};
+
+// One workgroup's segments of one grid: the whole forward program for the vertices of
+// segment group `seg_group` (kSegPerWG segments). Every thread of the block calls it.
 template <int K, int LC, int S, int kHm>
-__global__ __launch_bounds__(256) void water_forward_kernel(FwdParams p) {
+__device__ __forceinline__ void forward_segments(const FwdParams& p, long long seg_group, SegShared<K, LC, S>& sh) {
   constexpr int LR = 64 / LC;
   constexpr int KS = K / S;                       // vertices per lane in the epilogue
   constexpr int kSegPerWG = kWavesPerWG / S;
-  __shared__ WaveLds s_wave[kChunk];
-  __shared__ float4 s_rot[kWavesPerWG][LR * kChunk];  // (cos, sin, -sin, 0) per (row, wave)
-  // partial sums handed to the other wavefront(s) of the group: [group][sender][K-KS][8][64]
-  __shared__ float s_xchg[S > 1 ? kSegPerWG : 1][S > 1 ? S : 1][S > 1 ? (K - KS) * 8 * 64 : 1];
+  WaveLds* s_wave = sh.wave;
 
   const int lane = threadIdx.x & 63;
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
   const int grp = wid / S, sid = wid - grp * S;
-  const long long seg_global = (long long)blockIdx.x * kSegPerWG + grp;
+  const long long seg_global = seg_group * kSegPerWG + grp;
   const int segs_per_rg = (p.N + LC * K - 1) / (LC * K);
   const int n_rg = (p.N + LR - 1) / LR;
   const bool active = seg_global < (long long)segs_per_rg * n_rg;
@@ -128,9 +139,9 @@ __global__ __launch_bounds__(256) void water_forward_kernel(FwdParams p) {
     __syncthreads();
     stage_chunk(p, w0, nw, s_wave);
     __syncthreads();
-    build_rotations<LC, LR, S>(p, row_base, nw, sid, s_wave, s_rot[wid], lane);
+    build_rotations<LC, LR, S>(p, row_base, nw, sid, s_wave, sh.rot[wid], lane);
     __syncthreads();
-    const float4* rot_row = &s_rot[wid][lr * kChunk];
+    const float4* rot_row = &sh.rot[wid][lr * kChunk];
     if (sid >= nw) continue;
     for (int i = sid; i < nw; i += S) {
       const WaveLds& cur = s_wave[i];
@@ -163,7 +174,7 @@ __global__ __launch_bounds__(256) void water_forward_kernel(FwdParams p) {
     // S = 2: wavefront 0 keeps vertices [0, K/2), wavefront 1 keeps [K/2, K); each hands
     // the other half of its partial sums over through LDS and adds the half it receives
     __syncthreads();
-    float* mine = s_xchg[grp][sid];
+    float* mine = sh.xchg[grp][sid];
 #pragma unroll
     for (int j = 0; j < KS; ++j) {
       const int jo = sid == 0 ? j + KS : j;  // the step owned by the other wavefront
@@ -171,7 +182,7 @@ __global__ __launch_bounds__(256) void water_forward_kernel(FwdParams p) {
       for (int q = 0; q < 8; ++q) mine[(j * 8 + q) * 64 + lane] = acc[jo][q];
     }
     __syncthreads();
-    const float* theirs = s_xchg[grp][sid ^ 1];
+    const float* theirs = sh.xchg[grp][sid ^ 1];
 #pragma unroll
     for (int j = 0; j < KS; ++j) {
       const int jm = sid == 0 ? j : j + KS;  // my step
@@ -185,6 +196,73 @@ __global__ __launch_bounds__(256) void water_forward_kernel(FwdParams p) {
     epilogue_steps<K, LC, KS, 0, kHm>(p, g, acc, col0, row, active);
   else
     epilogue_steps<K, LC, KS, (S > 1 ? KS : 0), kHm>(p, g, acc, col0, row, active);
+}
+
+template <int K, int LC, int S, int kHm>
+__global__ __launch_bounds__(256) void water_forward_kernel(FwdParams p) {
+  __shared__ SegShared<K, LC, S> sh;
+  forward_segments<K, LC, S, kHm>(p, (long long)blockIdx.x, sh);
+}
+
+// ------------------------------------------------------------------ batched
+// Forward parameters of one patch of a batch (corner algebra done on the host).
+__device__ __forceinline__ FwdParams patch_params(const BatchArgs& a, const PatchDesc& pd) {
+  FwdParams p;
+  p.bl_x = pd.bl_x; p.bl_z = pd.bl_z;
+  p.du_x = pd.du_x; p.du_z = pd.du_z;
+  p.dv_x = pd.dv_x; p.dv_z = pd.dv_z;
+  p.duv_x = pd.duv_x; p.duv_z = pd.duv_z;
+  p.inv_nm1 = pd.inv_nm1;
+  p.time = pd.time + a.time_offset;
+  p.N = pd.N;
+  p.W = pd.W;
+  p.waves = a.waves + pd.wave_offset;
+  EpilogueArgs& e = p.ep;
+  e.cam[0] = pd.cam[0]; e.cam[1] = pd.cam[1]; e.cam[2] = pd.cam[2];
+  e.s_h = pd.s_h;
+  e.s_d = pd.s_d;
+  e.bounce = pd.bounce;
+  e.texels = a.texels;
+  e.tex_w = a.tex_w; e.tex_h = a.tex_h; e.tex_fmt = a.tex_fmt; e.tex_wrap = a.tex_wrap;
+  // texture-space scroll offset -s_h v t reduced to one wrap period in FP64
+  const double period = a.tex_wrap == WT_WRAP_REPEAT ? 1.0 : 2.0;
+  const double os = -(double)pd.s_h * (double)pd.scroll_x * p.time, ot = (double)pd.s_h * (double)pd.scroll_z * p.time;
+  e.hm_off_s = os - period * floor(os / period);
+  e.hm_off_t = ot - period * floor(ot / period);
+  const size_t base = 3 * (size_t)pd.vertex_offset;
+  e.pos = a.pos + base;
+  e.nrm = a.nrm + base;
+  e.tan = a.tan ? a.tan + base : nullptr;
+  e.bit = a.bit ? a.bit + base : nullptr;
+  return p;
+}
+
+// Persistent kernel over a batch of patches: each workgroup takes the next unit (one
+// workgroup's segments of one patch; units are sorted by cost, largest first) from a
+// device counter until the list is exhausted. The last workgroup to finish resets the
+// counters, so the launch can be replayed from a captured graph without a memset node.
+template <int K, int LC, int S, int kHm>
+__global__ __launch_bounds__(256) void water_forward_batched_kernel(BatchArgs a) {
+  __shared__ SegShared<K, LC, S> sh;
+  __shared__ int s_unit;
+  for (;;) {
+    if (threadIdx.x == 0) s_unit = (int)atomicAdd(&a.counters[0], 1u);
+    __syncthreads();
+    const int u = s_unit;
+    __syncthreads();  // every thread has read s_unit before it is overwritten
+    if (u >= a.n_units) break;
+    const BatchUnit unit = a.units[u];
+    const FwdParams p = patch_params(a, a.patches[unit.patch]);
+    forward_segments<K, LC, S, kHm>(p, (long long)unit.group, sh);
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const unsigned done = atomicAdd(&a.counters[1], 1u);
+    if (done == gridDim.x - 1) {
+      atomicExch(&a.counters[0], 0u);
+      atomicExch(&a.counters[1], 0u);
+    }
+  }
 }
 
 // ------------------------------------------------------------------ dispatch
@@ -222,6 +300,50 @@ hipError_t launch_forward(const FwdParams& p, const LaunchShape& s, hipStream_t 
     case 8: return launch_lc<8>(p, s.lanes_per_row, s.split, st);
     default: return launch_lc<16>(p, s.lanes_per_row, s.split, st);
   }
+}
+
+}  // namespace wt
+
+namespace wt {
+
+long long forward_segment_groups(int N, const LaunchShape& s) {
+  const int LC = s.lanes_per_row, K = s.steps, LR = 64 / LC;
+  const long long segs = (long long)((N + LC * K - 1) / (LC * K)) * ((N + LR - 1) / LR);
+  const int per_wg = kWavesPerWG / s.split;
+  return (segs + per_wg - 1) / per_wg;
+}
+
+// The batched kernel is built for two shapes: (K 8, LC 64, S 2) for grids wider than
+// 256 and (K 4, LC 32, S 1) otherwise; the batch picks one for all its patches.
+template <int K, int LC, int S>
+static hipError_t launch_batched_t(const BatchArgs& a, int grid, hipStream_t st) {
+  if (a.texels)
+    hipLaunchKernelGGL((water_forward_batched_kernel<K, LC, S, 1>), dim3(grid), dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL((water_forward_batched_kernel<K, LC, S, 0>), dim3(grid), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_forward_batched(const BatchArgs& a, const LaunchShape& s, int grid, hipStream_t st) {
+  if (s.steps == 8 && s.lanes_per_row == 64 && s.split == 2) return launch_batched_t<8, 64, 2>(a, grid, st);
+  return launch_batched_t<4, 32, 1>(a, grid, st);
+}
+
+template <int K, int LC, int S>
+static int occupancy_t(bool hm) {
+  int n = 0;
+  hipError_t e = hm ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, water_forward_batched_kernel<K, LC, S, 1>, 256, 0)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, water_forward_batched_kernel<K, LC, S, 0>, 256, 0);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return 1;
+  }
+  return n > 0 ? n : 1;
+}
+
+int batched_blocks_per_cu(const LaunchShape& s, bool hm) {
+  if (s.steps == 8 && s.lanes_per_row == 64 && s.split == 2) return occupancy_t<8, 64, 2>(hm);
+  return occupancy_t<4, 32, 1>(hm);
 }
 
 }  // namespace wt

@@ -31,4 +31,42 @@ struct LaunchShape {
 
 hipError_t launch_forward(const FwdParams& p, const LaunchShape& s, hipStream_t st);
 
+// One patch of a batch on the device: corner algebra precomputed on the host.
+struct PatchDesc {
+  double bl_x, bl_z, du_x, du_z, dv_x, dv_z, duv_x, duv_z, inv_nm1;
+  double time;
+  float cam[3];
+  float s_h, s_d, bounce, scroll_x, scroll_z;
+  int N, W, wave_offset, reserved;
+  long long vertex_offset;  // first vertex of the patch in the output arrays
+};
+
+// One workgroup's segment group of one patch.
+struct BatchUnit {
+  int patch;
+  int group;
+};
+
+struct BatchArgs {
+  const PatchDesc* patches;
+  const BatchUnit* units;
+  int n_units;
+  unsigned* counters;  // [0] next unit, [1] workgroups finished
+  double time_offset;  // added to every patch's time (frame clock)
+  const wt_gerstner_wave* waves;
+  const void* texels;
+  int tex_w, tex_h, tex_fmt, tex_wrap;
+  float* pos;
+  float* nrm;
+  float* tan;
+  float* bit;
+};
+
+// Segments a launch shape covers per workgroup for a grid of side N.
+long long forward_segment_groups(int N, const LaunchShape& s);
+// Launches the persistent batched kernel with `grid` workgroups.
+hipError_t launch_forward_batched(const BatchArgs& a, const LaunchShape& s, int grid, hipStream_t st);
+// Workgroups of the batched kernel resident per CU for this shape.
+int batched_blocks_per_cu(const LaunchShape& s, bool heightmap);
+
 }  // namespace wt

@@ -8,7 +8,7 @@ kernels in libwavetool.so.
 import ctypes
 
 from . import scenes
-from ._abi import (LaunchConfig, Camera, GerstnerWave, Heightmap, VertexBuffers, WaterParams, FitProblem, AdamConfig,
+from ._abi import (LaunchConfig, Camera, GerstnerWave, Heightmap, VertexBuffers, WaterParams, FitProblem, AdamConfig, Patch,
                    WtError, check, lib,
                    WT_OK, WT_ERR_INVALID_ARG, WT_ERR_NO_INTERSECTION, WT_ERR_DEGENERATE, WT_WRAP_REPEAT,
                    WT_WRAP_MIRRORED_REPEAT, WT_TEXEL_R8_UNORM, WT_TEXEL_R32F, WT_FLOP_PER_VERTEX_WAVE,
@@ -144,6 +144,77 @@ class Context:
         g = ctypes.c_void_p()
         self._check(lib().wt_graph_end(self.handle, ctypes.c_void_p(stream.cuda_stream), ctypes.byref(g)))
         return Graph(g)
+
+
+def patch(corners, cam_pos, N, wave_offset, wave_count, vertex_offset, time, sample_scale=0.0,
+          displacement_scale=0.0, bounce=0.0, scroll=(0.0, 0.0)):
+    p = Patch()
+    for i in range(4):
+        for k in range(4):
+            p.corners[i][k] = float(corners[i][k])
+    p.cameraPosition[:] = [float(v) for v in cam_pos]
+    p.gridLength, p.wave_offset, p.wave_count = int(N), int(wave_offset), int(wave_count)
+    p.vertex_offset = int(vertex_offset)
+    p.waveAnimationTimeInSeconds = float(time)
+    p.heightmapSampleScale = float(sample_scale)
+    p.heightmapDisplacementScale = float(displacement_scale)
+    p.verticalBounceWaveDisplacement = float(bounce)
+    p.heightmapScrollVelocity[:] = [float(scroll[0]), float(scroll[1])]
+    return p
+
+
+def batch_inputs(specs, amp=2.0, sample_scale=0.02, displacement_scale=0.05, scroll=(0.3, 0.0)):
+    """Patch structs and the concatenated wave array for specs from
+    scenes.batched_patches (each patch's corners from its own camera)."""
+    import numpy as np
+    patches, waves = [], []
+    woff = voff = 0
+    for s in specs:
+        corners = projector_corners(s["camera"], amp)
+        w = scenes.gen_waves(s["wave_seed"], s["W"])
+        patches.append(patch(corners, s["camera"][4:7], s["N"], woff, s["W"], voff, s["time"], sample_scale,
+                             displacement_scale, 0.0, scroll))
+        waves.append(w)
+        woff += s["W"]
+        voff += s["N"] * s["N"]
+    return patches, np.concatenate(waves, axis=0), voff
+
+
+class Batch:
+    """A batch of patches for the persistent batched forward kernel (wt_batch)."""
+
+    def __init__(self, ctx, patches):
+        self.ctx = ctx
+        arr = (Patch * len(patches))(*patches)
+        h = ctypes.c_void_p()
+        ctx._check(lib().wt_batch_create(ctx.handle, arr, len(patches), ctypes.byref(h)))
+        self.handle = h
+        v = ctypes.c_uint64()
+        check(lib().wt_batch_vertex_count(h, ctypes.byref(v)))
+        self.vertices = v.value
+
+    def displace(self, waves, pos, nrm, tan=None, bit=None, heightmap=None, hm_format=WT_TEXEL_R32F,
+                 hm_wrap=WT_WRAP_REPEAT, time_offset=0.0, stream=None):
+        vb = VertexBuffers(_ptr(pos), _ptr(nrm), _ptr(tan), _ptr(bit))
+        hm = None
+        if heightmap is not None:
+            h, w = heightmap.shape
+            hm = Heightmap(heightmap.data_ptr(), w, h, hm_format, hm_wrap)
+        st = lib().wt_water_displace_batched(self.ctx.handle, self.handle, _ptr(waves), int(waves.shape[0]),
+                                             ctypes.byref(hm) if hm is not None else None, ctypes.byref(vb),
+                                             float(time_offset), _stream(stream))
+        self.ctx._check(st)
+
+    def close(self):
+        if self.handle:
+            lib().wt_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def fit_problem(n, S, W, spacing, x0, z0, t0, dt, target, params, gravity=9.81):

@@ -95,6 +95,23 @@ class LaunchConfig(ctypes.Structure):
     ]
 
 
+class Patch(ctypes.Structure):
+    _fields_ = [
+        ("corners", (ctypes.c_double * 4) * 4),
+        ("cameraPosition", ctypes.c_float * 3),
+        ("gridLength", ctypes.c_uint32),
+        ("wave_offset", ctypes.c_uint32),
+        ("wave_count", ctypes.c_uint32),
+        ("vertex_offset", ctypes.c_uint64),
+        ("waveAnimationTimeInSeconds", ctypes.c_double),
+        ("heightmapSampleScale", ctypes.c_float),
+        ("heightmapDisplacementScale", ctypes.c_float),
+        ("verticalBounceWaveDisplacement", ctypes.c_float),
+        ("heightmapScrollVelocity", ctypes.c_float * 2),
+        ("reserved", ctypes.c_float),
+    ]
+
+
 class FitProblem(ctypes.Structure):
     _fields_ = [
         ("grid_side", ctypes.c_uint32),
@@ -144,6 +161,11 @@ PROTOTYPES = [
     ("wt_graph_end", ctypes.c_int, [_vp, _vp, ctypes.POINTER(_vp)]),
     ("wt_graph_launch", ctypes.c_int, [_vp, _vp]),
     ("wt_graph_destroy", None, [_vp]),
+    ("wt_batch_create", ctypes.c_int, [_vp, ctypes.POINTER(Patch), ctypes.c_uint32, ctypes.POINTER(_vp)]),
+    ("wt_batch_destroy", None, [_vp]),
+    ("wt_batch_vertex_count", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64)]),
+    ("wt_water_displace_batched", ctypes.c_int,
+     [_vp, _vp, _vp, ctypes.c_uint32, ctypes.POINTER(Heightmap), ctypes.POINTER(VertexBuffers), ctypes.c_double, _vp]),
     ("wt_fit_create", ctypes.c_int, [_vp, ctypes.POINTER(FitProblem), ctypes.c_uint32, ctypes.POINTER(_vp)]),
     ("wt_fit_destroy", None, [_vp]),
     ("wt_fit_reset", ctypes.c_int, [_vp, _vp]),
