@@ -1,16 +1,18 @@
 """Benchmark of the water-displacement stage on one MI355X.
 
-Workload (one step): the plain forward pass over the 1024 x 1024 projected grid with
-64 Gerstner waves and the 512 x 512 R32F value-noise heightmap, 240 frames at
-t = 10 + k/60 s, captured as one HIP graph of 240 kernel launches that re-fill the
+Headline workload (one step): the plain forward pass over the 1024 x 1024 projected
+grid with 64 Gerstner waves and the 512 x 512 R32F value-noise heightmap, 240 frames
+at t = 10 + k/60 s, captured as one HIP graph of 240 kernel launches that re-fill the
 same position / normal buffers (the per-frame vertex-buffer update of the
 reference's render loop). Inputs are resident in HBM before timing starts.
 
-Prints one JSON line: vertex-waves/s (V * W * frames / wall time), the roofline of
-the forward kernel (algorithmic 26 FLOP per vertex-wave / average launch duration
-from HIP events on the launch stream, against the 157.3 TFLOP/s FP32 peak), and the
-CPU baseline (the oracle's FP32 instantiation over std::threads on a bounded frame
-sample of the same workload).
+Prints one JSON line: vertex-waves/s (V * W * frames / wall time), the roofline of the
+forward kernel (algorithmic 26 FLOP per vertex-wave over the mean launch time from HIP
+events on the launch stream, against the 157.3 TFLOP/s FP32 peak), the CPU baseline
+(the oracle's FP32 instantiation over std::threads on a bounded frame sample of the
+same workload), and `extras` for the other workloads: the batched forward over 512
+uneven patches, one heightfield fit, and 256 uneven fits batched against the same fits
+launched one by one.
 """
 import argparse
 import json
@@ -25,18 +27,22 @@ for _p in (os.path.join(ROOT, "wave-tool_amd"), os.path.join(ROOT, "oracle")):
 
 METRIC = "Gerstner vertex-waves/s at 1 GPU; batched adjoint fits/s vs plain @1e-4 m"
 PEAK_FP32_TFLOPS = 157.3
-PEAK_HBM_GBS = 8000.0
+ADAM = dict(lr=1e-2, beta1=0.9, beta2=0.999, eps=1e-8)
 
 
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
+def cpu_threads():
+    return min(16, os.cpu_count() or 1)
+
+
 def cpu_baseline(corners, cam_pos, waves, texels, cfg, max_frames=16, budget_s=2.5):
     """FP32 oracle instantiation, all host threads (up to 16), frames of the same
     workload until the time budget or max_frames is reached."""
     import oracle as orc
-    threads = min(16, os.cpu_count() or 1)
+    threads = cpu_threads()
     N, W = cfg["N"], cfg["W"]
     frames = 0
     t_total = 0.0
@@ -59,30 +65,25 @@ def load_traffic():
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        d = json.load(f)
-    return d.get("hbm_bytes_per_launch")
+        return json.load(f).get("hbm_bytes_per_launch")
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-autotune", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (requested --gpus %d)" % args.gpus,
-              file=sys.stderr)
-        sys.exit(2)
+def timed(torch, stream, fn, reps=1):
+    """Wall time and HIP-event time (on `stream`) of reps calls of fn."""
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, e0.elapsed_time(e1) * 1e-3
 
-    import numpy as np
-    import torch
 
-    import wavetool_amd as wt
-    from wavetool_amd import scenes
-
-    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+# --------------------------------------------------------------------- config 1
+def bench_plain(torch, wt, scenes, args):
     dev = torch.device("cuda:0")
     cfg = scenes.plain_config(1)
     N, W, F = cfg["N"], cfg["W"], cfg["frames"]
@@ -97,7 +98,6 @@ def main():
     pos = torch.empty(V * 3, dtype=torch.float32, device=dev)
     nrm = torch.empty_like(pos)
     ctx = wt.Context()
-    shape = None
     if not args.no_autotune:
         shape, ms = ctx.autotune(N, W, True)
         log("autotuned launch shape %s (%.3f ms per frame)" % (shape, ms))
@@ -112,44 +112,209 @@ def main():
     graph = ctx.graph_end(stream)
     for _ in range(args.warmup):
         graph.launch(stream)
-    torch.cuda.synchronize()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        graph.launch(stream)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ev_ms = ev0.elapsed_time(ev1)
+    wall, ev = timed(torch, stream, lambda: graph.launch(stream), args.steps)
     graph.close()
     assert torch.isfinite(pos).all() and torch.isfinite(nrm).all()
-
     vw_per_step = V * W * F
-    ms_per_step = wall * 1e3 / args.steps
-    value = vw_per_step * args.steps / wall
-    launch_s = ev_ms * 1e-3 / (args.steps * F)
+    launch_s = ev / (args.steps * F)
     flop_per_launch = wt.WT_FLOP_PER_VERTEX_WAVE * V * W
-    achieved_tflops = flop_per_launch / launch_s / 1e12
-    traffic = load_traffic()
-    roofline = dict(bound="mfma", engine="valu", achieved=round(achieved_tflops, 3),
-                    peak=PEAK_FP32_TFLOPS, unit="TFLOP/s", frac=round(achieved_tflops / PEAK_FP32_TFLOPS, 4),
-                    traffic=traffic, launch_us=round(launch_s * 1e6, 3),
+    achieved = flop_per_launch / launch_s / 1e12
+    roofline = dict(bound="mfma", engine="valu", achieved=round(achieved, 3), peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
+                    frac=round(achieved / PEAK_FP32_TFLOPS, 4), traffic=load_traffic(),
+                    launch_us=round(launch_s * 1e6, 3),
                     hbm_write_GBs=round(wt.WT_BYTES_PER_VERTEX_PN * V / launch_s / 1e9, 1),
                     note="achieved = 26 FLOP x V x W per launch / mean launch time (HIP events on the launch "
-                         "stream); peak = MI355X FP32 vector")
-    out = dict(metric=METRIC, value=value, unit="vertex-waves/s", n_gpus=1, steps=args.steps, warmup=args.warmup,
-               ms_per_step=ms_per_step, higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f32",
-               data="synthetic",
+                         "stream); FP32 VALU kernel, peak = MI355X FP32 157.3 TFLOP/s")
+    out = dict(metric=METRIC, value=vw_per_step * args.steps / wall, unit="vertex-waves/s", n_gpus=1,
+               steps=args.steps, warmup=args.warmup, ms_per_step=wall * 1e3 / args.steps, higher_is_better=True,
+               scaling="weak", vs_baseline=None, dtype="f32", data="synthetic",
                config=dict(workload="config1: plain forward, 1024x1024 projected grid, 64 Gerstner waves, "
                                     "512x512 R32F value-noise heightmap, 240 frames per step (one HIP graph)",
                            grid=N, waves=W, frames_per_step=F, launch_shape=shape),
                roofline=roofline)
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(corners, cam_pos, waves_np, texels, cfg)
     ctx.close()
+    return out, (corners, cam_pos, waves_np, texels, cfg)
+
+
+# --------------------------------------------------------------------- config 2
+def bench_batched_forward(torch, wt, scenes, frames=120):
+    dev = torch.device("cuda:0")
+    specs = scenes.batched_patches(seed=2, count=512)
+    patches, waves_np, V = wt.batch_inputs(specs)
+    vw_frame = sum(s["N"] * s["N"] * s["W"] for s in specs)
+    ctx = wt.Context()
+    batch = wt.Batch(ctx, patches)
+    waves = torch.from_numpy(waves_np).to(dev)
+    hm = torch.from_numpy(scenes.value_noise(1, 512)).to(dev)
+    pos = torch.empty(V * 3, device=dev)
+    nrm = torch.empty_like(pos)
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    ctx.graph_begin(stream)
+    for f in range(frames):
+        batch.displace(waves, pos, nrm, heightmap=hm, time_offset=f / 60.0, stream=stream)
+    graph = ctx.graph_end(stream)
+    graph.launch(stream)
+    wall, _ = timed(torch, stream, lambda: graph.launch(stream))
+    graph.close()
+    # the same frame as one plain launch per patch (no graph), for the batching gain
+    params = [wt.water_params(wt.np_corners(p), s["camera"][4:7], s["N"], s["time"], 0.02, 0.05, 0.0, (0.3, 0.0))
+              for s, p in zip(specs, patches)]
+    views = [(waves[p.wave_offset:p.wave_offset + p.wave_count], pos[3 * p.vertex_offset:],
+              nrm[3 * p.vertex_offset:], p.wave_count) for p in patches]
+
+    torch.cuda.synchronize()
+    ctx.graph_begin(stream)
+    for prm, (w, P, Nn, W) in zip(params, views):
+        ctx.displace(prm, w, P, Nn, heightmap=hm, stream=stream, wave_count=W)
+    pgraph = ctx.graph_end(stream)
+    pgraph.launch(stream)
+    pwall, _ = timed(torch, stream, lambda: pgraph.launch(stream))
+    pgraph.close()
+    batch.close()
+    ctx.close()
+    batched = vw_frame * frames / wall
+    return dict(workload="config2: 512 patches (side 32-1024 log-uniform, 4-128 waves, own camera and t in "
+                         "[0, 3600) s), %d frames in one HIP graph of persistent launches" % frames,
+                value=batched, unit="vertex-waves/s", vertex_waves_per_frame=vw_frame,
+                vertices=V, ms_per_frame=wall * 1e3 / frames,
+                plain_per_patch=dict(value=vw_frame / pwall, unit="vertex-waves/s", ms_per_frame=pwall * 1e3,
+                                     note="one wt_water_displace launch per patch (autotune-free default shapes), "
+                                          "512 launches of one frame captured in a HIP graph"),
+                speedup_vs_plain=round(batched / (vw_frame / pwall), 2))
+
+
+# --------------------------------------------------------------------- configs 3 and 4
+def make_fits(torch, wt, scenes, ctx, specs, sigma=0.01):
+    """Device targets (synthesized from the hidden waves) and perturbed initial params."""
+    dev = torch.device("cuda:0")
+    problems, keep = [], []
+    for sp in specs:
+        prob = scenes.fit_problem(sp["seed"], n=sp["n"], S=sp["S"], W_hidden=sp["W"], W_fit=sp["W"])
+        hidden = torch.from_numpy(prob["hidden"].astype("float32")).to(dev)
+        target = torch.empty((sp["S"], sp["n"] * sp["n"], 3), device=dev)
+        hp = wt.fit_problem(sp["n"], sp["S"], sp["W"], prob["spacing"], prob["x0"], prob["z0"], prob["t0"], prob["dt"],
+                            None, hidden)
+        wt.synthesize(ctx, hp, sigma, sp["seed"], target)
+        params = torch.from_numpy(prob["init"].astype("float32")).to(dev)
+        problems.append(wt.fit_problem(sp["n"], sp["S"], sp["W"], prob["spacing"], prob["x0"], prob["z0"], prob["t0"],
+                                       prob["dt"], target, params))
+        keep.append((target, params, prob))
+    torch.cuda.synchronize()
+    return problems, keep
+
+
+def bench_fit_single(torch, wt, scenes, iterations=500, cpu=True):
+    ctx = wt.Context()
+    spec = dict(n=512, S=16, W=32, seed=3)
+    problems, keep = make_fits(torch, wt, scenes, ctx, [spec])
+    fit = wt.Fit(ctx, problems)
+    losses = torch.zeros(1, device="cuda")
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    ctx.graph_begin(stream)
+    fit.iterate(iterations, losses=losses, stream=stream, **ADAM)
+    graph = ctx.graph_end(stream)
+    wall, _ = timed(torch, stream, lambda: graph.launch(stream))
+    graph.close()
+    last = float(losses.item())
+    vws = spec["n"] ** 2 * spec["S"] * spec["W"]
+    out = dict(workload="config3: one fit, 512x512 grid at 0.5 m, 16 snapshots, 32 waves, %d Adam iterations "
+                        "(one HIP graph)" % iterations,
+               value=1.0 / wall, unit="fits/s", ms_per_iteration=wall * 1e3 / iterations, loss_last=last,
+               adjoint_algorithmic_TFLOPs=round(wt._abi.WT_ADJOINT_FLOP_PER_VERTEX_WAVE_SNAPSHOT * vws * iterations
+                                                / wall / 1e12, 2))
+    if cpu:
+        import oracle as orc
+        target_np = keep[0][0].cpu().numpy()
+        prob = keep[0][2]
+        fd = orc.FitData(prob["n"], prob["S"], prob["spacing"], prob["x0"], prob["z0"], prob["t0"], prob["dt"],
+                         target_np)
+        t0 = time.perf_counter()
+        orc.fit_loss_grad(fd, prob["init"], nthreads=cpu_threads())
+        it_s = time.perf_counter() - t0
+        out["cpu_baseline"] = dict(value=1.0 / (it_s * iterations), unit="fits/s", cores=cpu_threads(), kind="port",
+                                   sample="one FP64 oracle adjoint iteration (%.2f s wall), extrapolated to %d"
+                                          % (it_s, iterations))
+    fit.close()
+    ctx.close()
+    return out
+
+
+def bench_fit_batched(torch, wt, scenes, iterations=300, count=256):
+    ctx = wt.Context()
+    specs = scenes.fit_batch_specs(seed=4, count=count)
+    problems, keep = make_fits(torch, wt, scenes, ctx, specs)
+    init = [k[1].clone() for k in keep]
+    stream = torch.cuda.Stream()
+    # batched: one problem table, the iterations captured in one graph
+    fit = wt.Fit(ctx, problems)
+    torch.cuda.synchronize()
+    ctx.graph_begin(stream)
+    fit.iterate(iterations, stream=stream, **ADAM)
+    graph = ctx.graph_end(stream)
+    wall_b, _ = timed(torch, stream, lambda: graph.launch(stream))
+    graph.close()
+    batched_params = [k[1].clone() for k in keep]
+    fit.close()
+    # plain: the same problems one by one, each its own single-problem fit, eager launches
+    for k, p0 in zip(keep, init):
+        k[1].copy_(p0)
+    singles = [wt.Fit(ctx, [p]) for p in problems]
+    torch.cuda.synchronize()
+
+    def run_plain():
+        for f in singles:
+            f.iterate(iterations, stream=stream, **ADAM)
+    wall_p, _ = timed(torch, stream, run_plain)
+    same = all(torch.equal(k[1], b) for k, b in zip(keep, batched_params))
+    for f in singles:
+        f.close()
+    vws = sum(s["n"] ** 2 * s["S"] * s["W"] for s in specs)
+    ctx.close()
+    return dict(workload="config4: %d fits (side 64-512 at 0.5 m, 8-64 waves, 4-32 snapshots), %d Adam iterations; "
+                         "batched = one problem table, one HIP graph" % (count, iterations),
+                value=count / wall_b, unit="fits/s",
+                plain_one_by_one=dict(value=count / wall_p, unit="fits/s",
+                                      note="each problem its own single-problem fit, eager launches, in sequence"),
+                speedup_vs_plain=round(wall_p / wall_b, 2), results_identical=bool(same),
+                vertex_wave_snapshots_per_iteration=vws)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (requested --gpus %d)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import wavetool_amd as wt
+    from wavetool_amd import scenes
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    out, inputs = bench_plain(torch, wt, scenes, args)
+    log("config1 %.4g vertex-waves/s, %.2f us per frame" % (out["value"], out["roofline"]["launch_us"]))
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(*inputs)
+    if not args.no_extras:
+        extras = {}
+        extras["batched_forward"] = bench_batched_forward(torch, wt, scenes)
+        log("config2 %.4g vertex-waves/s" % extras["batched_forward"]["value"])
+        extras["fit_single"] = bench_fit_single(torch, wt, scenes, cpu=not args.no_cpu_baseline)
+        log("config3 %.4g fits/s" % extras["fit_single"]["value"])
+        extras["fit_batched"] = bench_fit_batched(torch, wt, scenes)
+        log("config4 %.4g fits/s (%.2fx one by one)" % (extras["fit_batched"]["value"],
+                                                         extras["fit_batched"]["speedup_vs_plain"]))
+        out["extras"] = extras
     print(json.dumps(out), flush=True)
 
 

@@ -163,6 +163,12 @@ def patch(corners, cam_pos, N, wave_offset, wave_count, vertex_offset, time, sam
     return p
 
 
+def np_corners(p):
+    """Corners [4][4] of a Patch as a numpy array."""
+    import numpy as np
+    return np.ctypeslib.as_array(p.corners).copy()
+
+
 def batch_inputs(specs, amp=2.0, sample_scale=0.02, displacement_scale=0.05, scroll=(0.3, 0.0)):
     """Patch structs and the concatenated wave array for specs from
     scenes.batched_patches (each patch's corners from its own camera)."""

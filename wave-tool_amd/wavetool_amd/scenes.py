@@ -192,6 +192,19 @@ def fit_problem(seed, n, S, W_hidden, W_fit, spacing=0.5, dt=0.25, noise=0.01):
                 init=init, noise_seed=seed * 7919 + 1)
 
 
+def fit_batch_specs(seed=4, count=256):
+    """Config 4: per problem grid side log-uniform in [64, 512] (0.5 m spacing), hidden and
+    fitted wave count U{8..64}, snapshots U{4..32} at 0.25 s, noise 0.01 m."""
+    rng = SplitMix64(seed)
+    out = []
+    for p in range(count):
+        n = int(round(math.exp(rng.uniform(math.log(64.0), math.log(512.0)))))
+        W = rng.randint(8, 64)
+        S = rng.randint(4, 32)
+        out.append(dict(n=n, W=W, S=S, seed=seed * 100003 + p))
+    return out
+
+
 def fit_target(prob, chunk=1 << 16):
     """Target displacement [S][n*n][3] float32 of the hidden waves plus noise, in FP64."""
     n, S = prob["n"], prob["S"]
