@@ -61,11 +61,16 @@ def cpu_baseline(corners, cam_pos, waves, texels, cfg, max_frames=16, budget_s=2
 
 
 def load_traffic():
-    path = os.path.join(ROOT, "profiles", "forward_config1_traffic.json")
+    """HBM bytes per forward launch from the committed FETCH_SIZE / WRITE_SIZE passes,
+    each scaled by the same counter's reading for a plain copy kernel of known size.
+    The write part (24 B per vertex) does not depend on the launch shape; the fetch part
+    is the heightmap, once per XCD L2."""
+    path = os.path.join(ROOT, "profiles", "water_forward_config1_traffic.json")
     if not os.path.exists(path):
-        return None
+        return None, None
     with open(path) as f:
-        return json.load(f).get("hbm_bytes_per_launch")
+        rec = json.load(f)
+    return rec.get("hbm_bytes_per_launch"), rec.get("launch_shape")
 
 
 def timed(torch, stream, fn, reps=1):
@@ -119,8 +124,9 @@ def bench_plain(torch, wt, scenes, args):
     launch_s = ev / (args.steps * F)
     flop_per_launch = wt.WT_FLOP_PER_VERTEX_WAVE * V * W
     achieved = flop_per_launch / launch_s / 1e12
+    traffic, traffic_shape = load_traffic()
     roofline = dict(bound="mfma", engine="valu", achieved=round(achieved, 3), peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
-                    frac=round(achieved / PEAK_FP32_TFLOPS, 4), traffic=load_traffic(),
+                    frac=round(achieved / PEAK_FP32_TFLOPS, 4), traffic=traffic, traffic_launch_shape=traffic_shape,
                     launch_us=round(launch_s * 1e6, 3),
                     hbm_write_GBs=round(wt.WT_BYTES_PER_VERTEX_PN * V / launch_s / 1e9, 1),
                     note="achieved = 26 FLOP x V x W per launch / mean launch time (HIP events on the launch "
