@@ -26,6 +26,18 @@ struct WaveLds {
 // argument of v_sin_f32 / v_cos_f32 stays in [0, 1] revolution.
 __device__ __forceinline__ float reduce_rev(double r) { return (float)__builtin_amdgcn_fract(r); }
 
+// cos and sin of 2 pi f for an FP64 fraction f in [0, 1): v_cos_f32 / v_sin_f32 of the
+// FP32-rounded fraction plus the first-order term of the rounding residual d (radians,
+// |d| < 4e-7), so the result has the accuracy of the hardware functions (about 1e-7
+// absolute) instead of that of an FP32 argument.
+__device__ __forceinline__ void sincos_rev(double f, float* c, float* s) {
+  const float a = (float)f;
+  const float d = (float)((f - (double)a) * kTwoPi);
+  const float ca = __builtin_amdgcn_cosf(a), sa = __builtin_amdgcn_sinf(a);
+  *c = fmaf(-d, sa, ca);
+  *s = fmaf(d, ca, sa);
+}
+
 __device__ __forceinline__ void stage_wave(const wt_gerstner_wave& g, double t, WaveLds* o) {
   const double w = (double)g.frequency_w, dx = (double)g.xzDirection_D[0], dz = (double)g.xzDirection_D[1];
   o->kx = w * dx * kInvTwoPi;

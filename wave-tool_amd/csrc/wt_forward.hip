@@ -52,22 +52,143 @@ __device__ __forceinline__ void stage_chunk(const FwdParams& p, int w0, int nw, 
   for (int t = threadIdx.x; t < nw; t += blockDim.x) stage_wave(p.waves[w0 + t], p.time, &s_wave[t]);
 }
 
-// Rotation per LC columns for (row of the segment, wave of the chunk), in FP64, for the
-// waves this wavefront sums (i = sid, sid + S, ...).
+// Per (row of a segment, wave of a wavefront) entry of the step table.
+//  rot:    cos, sin, -sin of the phase advance over LC columns, and 2 cos of it;
+//  seed:   phase of the segment's first column (revolutions, [0, 1)), the per-column
+//          advance reduced to [0, 1) and split into hi (a multiple of 2^-18, so that
+//          lc * hi is exact in FP32 for lc < 64) and lo;
+//  cc, cs: the wave's cos / sin coefficients (WaveLds::cc, ::cs).
+struct StepEntry {
+  float4 rot;
+  float4 seed;
+  float4 cc;
+  float4 cs;
+};
+
+// The three-term (Chebyshev) recurrence x[j+1] = 2 cos(d) x[j] - x[j-1] generates the
+// cos / sin sequence with one FMA per value instead of the rotation's two, but an FP32
+// rounding e of 2 cos(d) shifts the step by e / (2 |sin d|) and local rounding errors
+// grow by up to 1 / |sin d|; with |sin d| >= 0.3 and K <= 16 steps both stay below
+// 2e-6 rad / 6e-6 relative. A wave takes the recurrence for a segment when every row of
+// the segment satisfies this, i.e. when the step d = 2 pi f has f mod 1/2 in
+// [kChebMinRev, 1/2 - kChebMinRev], kChebMinRev = asin(0.3) / (2 pi).
+constexpr double kChebMinRev = 0.048493;
+
+// Build the step table of the waves this wavefront sums (i = sid, sid + S, ...) for the
+// LR rows of the segment starting at column col_base, in FP64, rounded once. Waves that
+// take the three-term recurrence come first; returns their number (wave-uniform).
 template <int LC, int LR, int S>
-__device__ __forceinline__ void build_rotations(const FwdParams& p, int row_base, int nw, int sid,
-                                                const WaveLds* s_wave, float4* s_rot, int lane) {
+__device__ __forceinline__ int build_table(const FwdParams& p, int row_base, int col_base, int nw, int sid,
+                                           const WaveLds* s_wave, StepEntry* tab, int lane) {
+  constexpr int kMine = kChunk / S;
   const int nmine = (nw - sid + S - 1) / S;
+  // which waves qualify for the recurrence on every row of the segment
+  bool cheb = lane < nmine;
+  if (cheb) {
+    const WaveLds& w = s_wave[sid + lane * S];
+#pragma unroll
+    for (int r = 0; r < LR; ++r) {
+      int row = row_base + r;
+      row = row < p.N ? row : p.N - 1;
+      const RowGeom g = row_geom(p, row);
+      const double st = fma(w.kx, g.xs, w.kz * g.zs) * (double)LC;
+      const double h = 2.0 * (st - floor(st));
+      const double f = 0.5 * (h - floor(h));  // step mod 1/2, in revolutions
+      cheb = cheb && f >= kChebMinRev && f <= 0.5 - kChebMinRev;
+    }
+  }
+  const uint64_t valid = nmine >= 64 ? ~0ull : ((1ull << nmine) - 1ull);
+  const uint64_t cmask = __ballot(cheb) & valid;
+  const int ncheb = __popcll(cmask);
   for (int e = lane; e < LR * nmine; e += 64) {
     const int r = e / nmine, m = e - r * nmine, i = sid + m * S;
+    const uint64_t mb = (1ull << m) - 1ull;
+    const int pos = ((cmask >> m) & 1ull) ? __popcll(cmask & mb) : ncheb + __popcll(~cmask & valid & mb);
     int row = row_base + r;
     row = row < p.N ? row : p.N - 1;
     const RowGeom g = row_geom(p, row);
-    const double step = (s_wave[i].kx * g.xs + s_wave[i].kz * g.zs) * (double)LC;
-    const double ang = (step - floor(step)) * kTwoPi;
-    double sn, cs;
-    sincos(ang, &sn, &cs);
-    s_rot[r * kChunk + i] = make_float4((float)cs, (float)sn, -(float)sn, 0.0f);
+    const WaveLds& w = s_wave[i];
+    const double q = fma(w.kx, g.xs, w.kz * g.zs);  // revolutions per column
+    const double qf = q - floor(q);
+    const double step = qf * (double)LC;
+    float cr, sr;
+    sincos_rev(step - floor(step), &cr, &sr);
+    const double ph = fma((double)col_base, q, fma(w.kx, g.xa, fma(w.kz, g.za, w.c0)));
+    const double qh = floor(qf * 262144.0) * (1.0 / 262144.0);
+    StepEntry t;
+    t.rot = make_float4(cr, sr, -sr, 2.0f * cr);
+    t.seed = make_float4((float)(ph - floor(ph)), (float)qh, (float)(qf - qh), 0.0f);
+    t.cc = make_float4(w.cc[0], w.cc[1], w.cc[2], w.cc[3]);
+    t.cs = make_float4(w.cs[0], w.cs[1], w.cs[2], w.cs[3]);
+    tab[r * kMine + pos] = t;
+  }
+  return ncheb;
+}
+
+// cos / sin of the phase of column col_base + lc from a seed entry: lc * hi is exact,
+// so only quantities below 3 revolutions are rounded.
+__device__ __forceinline__ void seed_trig(const float4& sd, float lcf, float* c, float* s) {
+  const float r = fmaf(lcf, sd.z, __builtin_amdgcn_fractf(lcf * sd.y) + sd.x);
+  *c = __builtin_amdgcn_cosf(r);
+  *s = __builtin_amdgcn_sinf(r);
+}
+
+// Partial sums of K consecutive vertices (LC columns apart) for one wave, steps
+// [J0, J1), rotation form: (c, s) holds step J0 on entry and step J1 on exit (J1 - 1
+// when J1 == K).
+template <int K>
+__device__ __forceinline__ void accumulate(float (&a)[8], float c, float s, const float4& cc, const float4& cs) {
+  a[0] = fmaf(cc.x, c, a[0]);
+  a[1] = fmaf(cc.y, c, a[1]);
+  a[2] = fmaf(cc.z, c, a[2]);
+  a[3] = fmaf(cc.w, c, a[3]);
+  a[4] = fmaf(cs.x, s, a[4]);
+  a[5] = fmaf(cs.y, s, a[5]);
+  a[6] = fmaf(cs.z, s, a[6]);
+  a[7] = fmaf(cs.w, s, a[7]);
+}
+
+template <int K, int J0, int J1>
+__device__ __forceinline__ void steps_rotation(float (&acc)[K][8], float& c, float& s, const float4& cc,
+                                               const float4& cs, const float4& rot) {
+#pragma unroll
+  for (int j = J0; j < J1; ++j) {
+    accumulate<K>(acc[j], c, s, cc, cs);
+    if (j + 1 < K) {
+      // (c, s) <- (c cr - s sr, s cr + c sr); -sr is staged so every op is a plain FMA
+      const float cn = fmaf(s, rot.z, c * rot.x);
+      s = fmaf(c, rot.y, s * rot.x);
+      c = cn;
+    }
+    // keep the recurrence interleaved with the sums: hoisting all K steps of it ahead
+    // of the FMAs would hold 2K extra live values
+    if ((j & 1) && j + 1 < J1) __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// The same steps with the three-term recurrence; step 0 -> 1 is a rotation, which also
+// sets the previous values (cm, sm).
+template <int K, int J0, int J1>
+__device__ __forceinline__ void steps_chebyshev(float (&acc)[K][8], float& c, float& s, float& cm, float& sm,
+                                                const float4& cc, const float4& cs, const float4& rot) {
+#pragma unroll
+  for (int j = J0; j < J1; ++j) {
+    accumulate<K>(acc[j], c, s, cc, cs);
+    if (j + 1 < K) {
+      float cn, sn;
+      if (j == 0) {
+        cn = fmaf(s, rot.z, c * rot.x);
+        sn = fmaf(c, rot.y, s * rot.x);
+      } else {
+        cn = fmaf(rot.w, c, -cm);
+        sn = fmaf(rot.w, s, -sm);
+      }
+      cm = c;
+      sm = s;
+      c = cn;
+      s = sn;
+    }
+    if ((j & 1) && j + 1 < J1) __builtin_amdgcn_sched_barrier(0);
   }
 }
 
@@ -91,17 +212,26 @@ __device__ __forceinline__ void epilogue_steps(const FwdParams& p, const RowGeom
   }
 }
 
-// LDS of one workgroup of the forward kernels.
+// LDS of one workgroup of the forward kernels. The wave constants and step tables are
+// dead once the main loop ends, so the S = 2 exchange of partial sums reuses their space.
 template <int K, int LC, int S>
 struct SegShared {
   static constexpr int LR = 64 / LC;
   static constexpr int KS = K / S;
   static constexpr int kSegPerWG = kWavesPerWG / S;
-  WaveLds wave[kChunk];
-  float4 rot[kWavesPerWG][LR * kChunk];  // (cos, sin, -sin, 0) per (row, wave)
+  static constexpr int kTab = LR * (kChunk / S);  // step-table entries per wavefront
+  static constexpr size_t kLoopBytes = sizeof(WaveLds) * kChunk + sizeof(StepEntry) * kWavesPerWG * kTab;
   // partial sums handed to the other wavefront of the pair: [pair][sender][K-KS][8][64]
-  float xchg[S > 1 ? kSegPerWG : 1][S > 1 ? S : 1][S > 1 ? (K - KS) * 8 * 64 : 1];
+  static constexpr size_t kXchgFloats = (size_t)(K - KS) * 8 * 64;
+  static constexpr size_t kXchgBytes = S > 1 ? sizeof(float) * kSegPerWG * S * kXchgFloats : 0;
+  alignas(16) unsigned char raw[kLoopBytes > kXchgBytes ? kLoopBytes : kXchgBytes];
+  __device__ WaveLds* wave() { return reinterpret_cast<WaveLds*>(raw); }
+  __device__ StepEntry* tab(int wid) {
+    return reinterpret_cast<StepEntry*>(raw + sizeof(WaveLds) * kChunk) + wid * kTab;
+  }
+  __device__ float* xchg(int grp, int sender) { return reinterpret_cast<float*>(raw) + (grp * S + sender) * kXchgFloats; }
 };
+static_assert(sizeof(WaveLds) * kChunk % 16 == 0, "step table alignment");
 
 // One workgroup's segments of one grid: the whole forward program for the vertices of
 // segment group `seg_group` (kSegPerWG segments). Every thread of the block calls it.
@@ -110,7 +240,7 @@ __device__ __forceinline__ void forward_segments(const FwdParams& p, long long s
   constexpr int LR = 64 / LC;
   constexpr int KS = K / S;                       // vertices per lane in the epilogue
   constexpr int kSegPerWG = kWavesPerWG / S;
-  WaveLds* s_wave = sh.wave;
+  WaveLds* s_wave = sh.wave();
 
   const int lane = threadIdx.x & 63;
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
@@ -125,8 +255,8 @@ __device__ __forceinline__ void forward_segments(const FwdParams& p, long long s
   const int lr = lane / LC, lc = lane - lr * LC;
   const int row = row_base + lr;
   const int col0 = seg * LC * K + lc;
-  const RowGeom g = row_geom(p, row < p.N ? row : p.N - 1);
-  const double x0 = fma((double)col0, g.xs, g.xa), z0 = fma((double)col0, g.zs, g.za);
+  const int col_base = seg * LC * K;
+  const float lcf = (float)lc;
 
   float acc[K][8];
 #pragma unroll
@@ -139,42 +269,52 @@ __device__ __forceinline__ void forward_segments(const FwdParams& p, long long s
     __syncthreads();
     stage_chunk(p, w0, nw, s_wave);
     __syncthreads();
-    build_rotations<LC, LR, S>(p, row_base, nw, sid, s_wave, sh.rot[wid], lane);
+    const int ncheb = build_table<LC, LR, S>(p, row_base, col_base, nw, sid, s_wave, sh.tab(wid), lane);
     __syncthreads();
-    const float4* rot_row = &sh.rot[wid][lr * kChunk];
     if (sid >= nw) continue;
-    for (int i = sid; i < nw; i += S) {
-      const WaveLds& cur = s_wave[i];
-      const float4 rot = rot_row[i];
-      const float r = reduce_rev(fma(cur.kx, x0, fma(cur.kz, z0, cur.c0)));
-      float c = __builtin_amdgcn_cosf(r), s = __builtin_amdgcn_sinf(r);
-      const float c0 = cur.cc[0], c1 = cur.cc[1], c2 = cur.cc[2], c3 = cur.cc[3];
-      const float s0 = cur.cs[0], s1 = cur.cs[1], s2 = cur.cs[2], s3 = cur.cs[3];
-#pragma unroll
-      for (int j = 0; j < K; ++j) {
-        acc[j][0] = fmaf(c0, c, acc[j][0]);
-        acc[j][1] = fmaf(c1, c, acc[j][1]);
-        acc[j][2] = fmaf(c2, c, acc[j][2]);
-        acc[j][3] = fmaf(c3, c, acc[j][3]);
-        acc[j][4] = fmaf(s0, s, acc[j][4]);
-        acc[j][5] = fmaf(s1, s, acc[j][5]);
-        acc[j][6] = fmaf(s2, s, acc[j][6]);
-        acc[j][7] = fmaf(s3, s, acc[j][7]);
-        if (j + 1 < K) {
-          // (c, s) <- (c cr - s sr, s cr + c sr); -sr is staged so every op is a plain FMA
-          const float cn = fmaf(s, rot.z, c * rot.x);
-          s = fmaf(c, rot.y, s * rot.x);
-          c = cn;
-        }
-      }
+    const StepEntry* tab = sh.tab(wid) + lr * (kChunk / S);
+    // Software pipeline over this wavefront's table entries (three-term recurrence
+    // first, then rotation): the seed of entry k + 1 is formed between the two halves
+    // of entry k's steps (scheduling barriers keep the compiler from hoisting its
+    // dependent chain back to the top of the iteration).
+    const int nmine = (nw - sid + S - 1) / S;
+    float c, s, cm = 0.0f, sm = 0.0f;
+    seed_trig(tab[0].seed, lcf, &c, &s);
+    int k = 0;
+    for (; k < ncheb; ++k) {
+      const StepEntry& t = tab[k];
+      const float4 rot = t.rot, cc = t.cc, cs = t.cs;
+      const float4 sd = tab[k + 1 < nmine ? k + 1 : k].seed;
+      float cn, sn;
+      steps_chebyshev<K, 0, K / 2>(acc, c, s, cm, sm, cc, cs, rot);
+      __builtin_amdgcn_sched_barrier(0);
+      seed_trig(sd, lcf, &cn, &sn);
+      __builtin_amdgcn_sched_barrier(0);
+      steps_chebyshev<K, K / 2, K>(acc, c, s, cm, sm, cc, cs, rot);
+      c = cn;
+      s = sn;
+    }
+    for (; k < nmine; ++k) {
+      const StepEntry& t = tab[k];
+      const float4 rot = t.rot, cc = t.cc, cs = t.cs;
+      const float4 sd = tab[k + 1 < nmine ? k + 1 : k].seed;
+      float cn, sn;
+      steps_rotation<K, 0, K / 2>(acc, c, s, cc, cs, rot);
+      __builtin_amdgcn_sched_barrier(0);
+      seed_trig(sd, lcf, &cn, &sn);
+      __builtin_amdgcn_sched_barrier(0);
+      steps_rotation<K, K / 2, K>(acc, c, s, cc, cs, rot);
+      c = cn;
+      s = sn;
     }
   }
+  const RowGeom g = row_geom(p, row < p.N ? row : p.N - 1);
 
   if (S > 1) {
     // S = 2: wavefront 0 keeps vertices [0, K/2), wavefront 1 keeps [K/2, K); each hands
     // the other half of its partial sums over through LDS and adds the half it receives
     __syncthreads();
-    float* mine = sh.xchg[grp][sid];
+    float* mine = sh.xchg(grp, sid);
 #pragma unroll
     for (int j = 0; j < KS; ++j) {
       const int jo = sid == 0 ? j + KS : j;  // the step owned by the other wavefront
@@ -182,7 +322,7 @@ __device__ __forceinline__ void forward_segments(const FwdParams& p, long long s
       for (int q = 0; q < 8; ++q) mine[(j * 8 + q) * 64 + lane] = acc[jo][q];
     }
     __syncthreads();
-    const float* theirs = sh.xchg[grp][sid ^ 1];
+    const float* theirs = sh.xchg(grp, sid ^ 1);
 #pragma unroll
     for (int j = 0; j < KS; ++j) {
       const int jm = sid == 0 ? j : j + KS;  // my step
