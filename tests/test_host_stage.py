@@ -1,0 +1,67 @@
+"""The C++ host water stage (wave-tool_amd/host): the render loop's per-frame update over
+the C ABI, on the reference's default scene, against the golden fixture; and the water
+grid's triangle index list (src/program.cpp:686-717)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import NRM_TOL, POS_TOL, compare, near_texel_edge
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CHECK = os.path.join(ROOT, "wave-tool_amd", "host", "water_stage_check")
+GOLDEN = os.path.join(ROOT, "tests", "golden", "default_scene.npz")
+
+
+def reference_indices(n):
+    """src/program.cpp:686-717 restated: ids row-major from the bottom-left, per cell
+    (r,c), (r+1,c+1), (r+1,c) then (r,c), (r,c+1), (r+1,c+1)."""
+    ids = np.arange(n * n).reshape(n, n)
+    out = []
+    for r in range(n - 1):
+        for c in range(n - 1):
+            out += [ids[r, c], ids[r + 1, c + 1], ids[r + 1, c], ids[r, c], ids[r, c + 1], ids[r + 1, c + 1]]
+    return np.array(out)
+
+
+@pytest.mark.parametrize("n", [2, 3, 17])
+def test_triangle_indices(n):
+    if not os.path.exists(CHECK):
+        pytest.fail("water_stage_check is not built (run __graft_entry__.build())")
+    out = subprocess.run([CHECK, "--indices", str(n)], capture_output=True, text=True, check=True).stdout.split()
+    got = np.array([int(v) for v in out])
+    assert got.size == 6 * (n - 1) ** 2
+    assert np.array_equal(got, reference_indices(n))
+    # every triangle is counter-clockwise seen from +y on the undisplaced grid (x = col, z = -row)
+    tri = got.reshape(-1, 3)
+    x, z = tri % n, -(tri // n)
+    area = (x[:, 1] - x[:, 0]) * (z[:, 2] - z[:, 0]) - (x[:, 2] - x[:, 0]) * (z[:, 1] - z[:, 0])
+    assert (area < 0).all() or (area > 0).all()
+
+
+@pytest.mark.gpu
+def test_host_stage_default_scene(tmp_path):
+    out = tmp_path / "stage.bin"
+    print("running", CHECK, flush=True)
+    r = subprocess.run([CHECK, str(out)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    raw = out.read_bytes()
+    N = 17
+    nf = N * N * 3
+    corners = np.frombuffer(raw[:128], dtype=np.float64).reshape(4, 4)
+    frames = np.frombuffer(raw[128:], dtype=np.float32).reshape(3, 4, N * N, 3)
+    with np.load(GOLDEN, allow_pickle=False) as z:
+        g = {k: z[k] for k in z.files}
+    # the stage forms the slab amplitude in FP32 like render-engine.cpp:803; the fixture in FP64
+    assert np.abs(corners - g["corners"]).max() <= 1e-5 * np.abs(g["corners"]).max()
+    assert frames.size == 3 * 4 * nf
+    for f, t in enumerate((0.0, 10.0, 3600.0)):
+        got = dict(zip(("pos", "nrm", "tan", "bit"), frames[f]))
+        ref = {k: g["t%g_%s" % (t, k)] for k in ("pos", "nrm", "tan", "bit")}
+        edge = near_texel_edge(ref["pos"], 0.02, (0.0, 0.0), t, 16, 16)
+        res = compare(got, ref, (0.0, 4.0, 70.0), edge_mask=edge)
+        print(t, res, flush=True)
+        assert res["pos_max"] <= POS_TOL and res["nrm_max"] <= NRM_TOL
+        assert res["tan_max"] <= NRM_TOL and res["bit_max"] <= NRM_TOL
