@@ -6,28 +6,30 @@
 // d = (Q A Dx cos theta, A sin theta, Q A Dz cos theta) (water-grid.vert:79-81).
 // Loss = 1 / (2 S V) * sum over snapshots and grid points of |d - target|^2.
 //
-// adjoint_kernel: one thread per grid point, all S snapshots in registers.
+// adjoint_kernel<SMAX, KP>: KP grid points per thread (256 KP per block), all S <= SMAX
+//   snapshots of each in registers; one instantiation per snapshot class (8, 16, 32),
+//   so a batch of mixed problems runs one launch per class present.
 //   forward: per wave, seed (cos, sin) from the FP64-reduced phase at t0 and step the
 //     snapshots with a rotation by omega dt; accumulate d per snapshot.
 //   residual r = d - target, loss contribution 1/2 |r|^2.
 //   backward: per wave, re-seed and step again (recomputing the phases is cheaper than
-//     storing them: 4 VALU ops per snapshot against an 8-byte HBM round trip), with
-//     nine running sums over snapshots, from which the five parameter derivatives of
-//     this grid point follow; then a fixed-order wavefront reduction (xor shuffles),
-//     a fixed-order sum over the 4 wavefronts, and one partial per (parameter, block)
-//     in global memory.
+//     storing them: 4 VALU ops per snapshot against an 8-byte HBM round trip), with six
+//     running sums over snapshots plus three running sums of those sums (the
+//     time-weighted sums follow by summation by parts: sum_s s f_s = S U - V), from
+//     which the five parameter derivatives of each grid point follow; the thread's KP
+//     points are added, then a fixed-order DPP wavefront reduction, a fixed-order sum
+//     over the 4 wavefronts, and one partial per (parameter, block) in global memory.
 // reduce_adam_kernel: one workgroup per (problem, parameter row): fixed-order sum of
 //   the block partials, scaling to the mean misfit, and either an Adam step on the
 //   parameter or a plain gradient / loss write-out.
-// Both kernels walk a table of problems, so one launch covers a whole batch; a single
-// problem is a table of one.
+// Both kernels find their problem through a block (row) -> problem map built at
+// creation, so one launch per snapshot class covers a whole batch; a single problem is a
+// table of one.
 #include <hip/hip_runtime.h>
 
 #include "wt_fit.hpp"
 
 namespace wt {
-
-constexpr int kFitThreads = 256;
 
 struct FitWaveLds {
   double kx, kz, c0;   // revolutions per metre along x / z, phase at t0 (fract)
@@ -37,21 +39,20 @@ struct FitWaveLds {
   float w, wL, om2L;   // w, w / L, omega / (2 L)
 };
 
-__device__ __forceinline__ int find_problem(const FitDesc* __restrict__ probs, int count, long long key,
-                                            bool by_param) {
-  int lo = 0, hi = count - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    const long long start = by_param ? probs[mid].row_offset : probs[mid].block_offset;
-    if (start <= key) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
+// Sum over the 64 lanes of a wavefront with DPP moves (VALU only, fixed order); the
+// total lands in lane 63. quad_perm [1,0,3,2] and [2,3,0,1] give quad sums,
+// row_half_mirror and row_mirror row sums, row_bcast:15 / :31 the rest.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_term(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, false));
 }
-
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+__device__ __forceinline__ float wave_sum63(float v) {
+  v += dpp_term<0xB1, 0xf>(v);
+  v += dpp_term<0x4E, 0xf>(v);
+  v += dpp_term<0x141, 0xf>(v);
+  v += dpp_term<0x140, 0xf>(v);
+  v += dpp_term<0x142, 0xa>(v);
+  v += dpp_term<0x143, 0xc>(v);
   return v;
 }
 
@@ -83,110 +84,131 @@ __device__ __forceinline__ void stage_fit_wave(const FitDesc& P, int i, FitWaveL
   o->om2L = (float)(om / (2.0 * L));
 }
 
-template <int SMAX>
+template <int SMAX, int KP>
 __device__ __forceinline__ void adjoint_body(const FitDesc& P, int local_block, const FitWaveLds* s_wave,
                                              float* s_red) {
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int V = P.n * P.n;
-  const int v = local_block * kFitThreads + tid;
-  const bool active = v < V;
-  const int vv = active ? v : 0;
-  const double x = P.x0 + (double)(vv % P.n) * P.spacing;
-  const double z = P.z0 + (double)(vv / P.n) * P.spacing;
   const int S = P.S;
-
-  float r[SMAX][3];
+  bool active[KP];
+  int vv[KP];
+  double x[KP], z[KP];
 #pragma unroll
-  for (int s = 0; s < SMAX; ++s) r[s][0] = r[s][1] = r[s][2] = 0.0f;
+  for (int k = 0; k < KP; ++k) {
+    const int v = (local_block * KP + k) * kFitThreads + tid;
+    active[k] = v < V;
+    vv[k] = active[k] ? v : 0;
+    x[k] = P.x0 + (double)(vv[k] % P.n) * P.spacing;
+    z[k] = P.z0 + (double)(vv[k] / P.n) * P.spacing;
+  }
+
+  float r[KP][SMAX][3];
+#pragma unroll
+  for (int k = 0; k < KP; ++k)
+#pragma unroll
+    for (int s = 0; s < SMAX; ++s) r[k][s][0] = r[k][s][1] = r[k][s][2] = 0.0f;
 
   // forward: d per snapshot
   for (int i = 0; i < P.W; ++i) {
     const FitWaveLds& wv = s_wave[i];
-    const float ph = (float)__builtin_amdgcn_fract(fma(wv.kx, x, fma(wv.kz, z, wv.c0)));
-    float c = __builtin_amdgcn_cosf(ph), sn = __builtin_amdgcn_sinf(ph);
     const float qadx = wv.qadx, qadz = wv.qadz, A = wv.A, rc = wv.rc, rs = wv.rs, nrs = -wv.rs;
 #pragma unroll
-    for (int s = 0; s < SMAX; ++s) {
-      if (s < S) {
-        r[s][0] = fmaf(qadx, c, r[s][0]);
-        r[s][1] = fmaf(A, sn, r[s][1]);
-        r[s][2] = fmaf(qadz, c, r[s][2]);
-        const float cn = fmaf(sn, nrs, c * rc);
-        sn = fmaf(c, rs, sn * rc);
-        c = cn;
+    for (int k = 0; k < KP; ++k) {
+      const float ph = (float)__builtin_amdgcn_fract(fma(wv.kx, x[k], fma(wv.kz, z[k], wv.c0)));
+      float c = __builtin_amdgcn_cosf(ph), sn = __builtin_amdgcn_sinf(ph);
+#pragma unroll
+      for (int s = 0; s < SMAX; ++s) {
+        if (s < S) {
+          r[k][s][0] = fmaf(qadx, c, r[k][s][0]);
+          r[k][s][1] = fmaf(A, sn, r[k][s][1]);
+          r[k][s][2] = fmaf(qadz, c, r[k][s][2]);
+          const float cn = fmaf(sn, nrs, c * rc);
+          sn = fmaf(c, rs, sn * rc);
+          c = cn;
+        }
       }
     }
   }
   // residuals and loss
   float loss = 0.0f;
 #pragma unroll
-  for (int s = 0; s < SMAX; ++s) {
-    if (s < S) {
-      const float* tg = P.target + ((size_t)s * V + vv) * 3;
-      const float t0 = tg[0], t1 = tg[1], t2 = tg[2];
-      r[s][0] = active ? r[s][0] - t0 : 0.0f;
-      r[s][1] = active ? r[s][1] - t1 : 0.0f;
-      r[s][2] = active ? r[s][2] - t2 : 0.0f;
-      loss = fmaf(0.5f * r[s][0], r[s][0], loss);
-      loss = fmaf(0.5f * r[s][1], r[s][1], loss);
-      loss = fmaf(0.5f * r[s][2], r[s][2], loss);
-    }
-  }
-  const float lsum = wave_sum(loss);
-  if (lane == 0) s_red[(P.W * 5) * 4 + wid] = lsum;
-
-  // backward: nine sums over snapshots per wave, then the five derivatives
-  const float xf = (float)(x - P.x0), zf = (float)(z - P.z0);  // local coordinates for the D.x terms
-  const float x0f = (float)P.x0, z0f = (float)P.z0;
-  for (int i = 0; i < P.W; ++i) {
-    const FitWaveLds& wv = s_wave[i];
-    const float ph = (float)__builtin_amdgcn_fract(fma(wv.kx, x, fma(wv.kz, z, wv.c0)));
-    float c = __builtin_amdgcn_cosf(ph), sn = __builtin_amdgcn_sinf(ph);
-    const float rc = wv.rc, rs = wv.rs, nrs = -wv.rs;
-    float sxc = 0, szc = 0, syc = 0, sxs = 0, szs = 0, sys = 0, tsyc = 0, tsxs = 0, tszs = 0;
-    float t = 0.0f;  // time since t0 of snapshot s
-    const float dt = (float)P.dt;
+  for (int k = 0; k < KP; ++k) {
 #pragma unroll
     for (int s = 0; s < SMAX; ++s) {
       if (s < S) {
-        const float rx = r[s][0], ry = r[s][1], rz = r[s][2];
-        sxc = fmaf(rx, c, sxc);
-        szc = fmaf(rz, c, szc);
-        syc = fmaf(ry, c, syc);
-        sxs = fmaf(rx, sn, sxs);
-        szs = fmaf(rz, sn, szs);
-        sys = fmaf(ry, sn, sys);
-        const float tc = t * c, ts = t * sn;
-        tsyc = fmaf(ry, tc, tsyc);
-        tsxs = fmaf(rx, ts, tsxs);
-        tszs = fmaf(rz, ts, tszs);
-        const float cn = fmaf(sn, nrs, c * rc);
-        sn = fmaf(c, rs, sn * rc);
-        c = cn;
-        t += dt;
+        const float* tg = P.target + ((size_t)s * V + vv[k]) * 3;
+        const float t0 = tg[0], t1 = tg[1], t2 = tg[2];
+        r[k][s][0] = active[k] ? r[k][s][0] - t0 : 0.0f;
+        r[k][s][1] = active[k] ? r[k][s][1] - t1 : 0.0f;
+        r[k][s][2] = active[k] ? r[k][s][2] - t2 : 0.0f;
+        loss = fmaf(0.5f * r[k][s][0], r[k][s][0], loss);
+        loss = fmaf(0.5f * r[k][s][1], r[k][s][1], loss);
+        loss = fmaf(0.5f * r[k][s][2], r[k][s][2], loss);
       }
     }
+  }
+  const float lsum = wave_sum63(loss);
+  if (lane == 63) s_red[(P.W * 5) * 4 + wid] = lsum;
+
+  // backward: per wave, six sums over snapshots and three sums of their running values
+  // per point, then the five derivatives, added over the thread's points
+  const float x0f = (float)P.x0, z0f = (float)P.z0;
+  const float dtf = (float)P.dt, Sf = (float)S, t0f = (float)P.t0;
+  for (int i = 0; i < P.W; ++i) {
+    const FitWaveLds& wv = s_wave[i];
+    const float rc = wv.rc, rs = wv.rs, nrs = -wv.rs;
     const float A = wv.A, Q = wv.Q, Dx = wv.Dx, Dz = wv.Dz;
-    const float sec = Dx * sxc + Dz * szc;      // sum of (r . D) cos
-    const float sepc = -Dz * sxc + Dx * szc;    // sum of (r . D-perp) cos
-    const float g0 = A * (syc - Q * (Dx * sxs + Dz * szs));          // sum of r . dd/dtheta
-    const float gt = A * (tsyc - Q * (Dx * tsxs + Dz * tszs));       // sum of (t - t0) r . dd/dtheta
-    const float g0t = fmaf((float)P.t0, g0, gt);                     // sum of t r . dd/dtheta
-    // D.x and D-perp.x split into the problem origin term and the local term
-    const float dotx = fmaf(Dx, xf, Dz * zf) + fmaf(Dx, x0f, Dz * z0f);
-    const float perpx = fmaf(-Dz, xf, Dx * zf) + fmaf(-Dz, x0f, Dx * z0f);
-    float gA = fmaf(Q, sec, sys);
-    float gL = -wv.wL * dotx * g0 - wv.om2L * g0t;
-    float gAl = fmaf(wv.w * perpx, g0, Q * A * sepc);
-    float gQ = A * sec;
-    float gP = g0;
-    if (!active) gA = gL = gAl = gQ = gP = 0.0f;
-    gA = wave_sum(gA);
-    gL = wave_sum(gL);
-    gAl = wave_sum(gAl);
-    gQ = wave_sum(gQ);
-    gP = wave_sum(gP);
-    if (lane == 0) {
+    const float origin_dot = fmaf(Dx, x0f, Dz * z0f), origin_perp = fmaf(-Dz, x0f, Dx * z0f);
+    float gA = 0.0f, gL = 0.0f, gAl = 0.0f, gQ = 0.0f, gP = 0.0f;
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+      const float ph = (float)__builtin_amdgcn_fract(fma(wv.kx, x[k], fma(wv.kz, z[k], wv.c0)));
+      float c = __builtin_amdgcn_cosf(ph), sn = __builtin_amdgcn_sinf(ph);
+      float sxc = 0, szc = 0, syc = 0, sxs = 0, szs = 0, sys = 0, vyc = 0, vxs = 0, vzs = 0;
+#pragma unroll
+      for (int s = 0; s < SMAX; ++s) {
+        if (s < S) {
+          const float rx = r[k][s][0], ry = r[k][s][1], rz = r[k][s][2];
+          sxc = fmaf(rx, c, sxc);
+          szc = fmaf(rz, c, szc);
+          syc = fmaf(ry, c, syc);
+          sxs = fmaf(rx, sn, sxs);
+          szs = fmaf(rz, sn, szs);
+          sys = fmaf(ry, sn, sys);
+          vyc += syc;
+          vxs += sxs;
+          vzs += szs;
+          const float cn = fmaf(sn, nrs, c * rc);
+          sn = fmaf(c, rs, sn * rc);
+          c = cn;
+        }
+      }
+      // time-weighted sums: sum_s (s dt) f_s = dt (S U - V)
+      const float tsyc = dtf * fmaf(Sf, syc, -vyc);
+      const float tsxs = dtf * fmaf(Sf, sxs, -vxs);
+      const float tszs = dtf * fmaf(Sf, szs, -vzs);
+      const float sec = Dx * sxc + Dz * szc;    // sum of (r . D) cos
+      const float sepc = -Dz * sxc + Dx * szc;  // sum of (r . D-perp) cos
+      const float g0 = A * (syc - Q * (Dx * sxs + Dz * szs));     // sum of r . dd/dtheta
+      const float gt = A * (tsyc - Q * (Dx * tsxs + Dz * tszs));  // sum of (t - t0) r . dd/dtheta
+      const float g0t = fmaf(t0f, g0, gt);                        // sum of t r . dd/dtheta
+      // D.x and D-perp.x split into the problem origin term and the local term
+      const float xf = (float)(x[k] - P.x0), zf = (float)(z[k] - P.z0);
+      const float dotx = fmaf(Dx, xf, Dz * zf) + origin_dot;
+      const float perpx = fmaf(-Dz, xf, Dx * zf) + origin_perp;
+      // inactive points have zero residuals, so every sum above is zero for them
+      gA += fmaf(Q, sec, sys);
+      gL += -wv.wL * dotx * g0 - wv.om2L * g0t;
+      gAl += fmaf(wv.w * perpx, g0, Q * A * sepc);
+      gQ += A * sec;
+      gP += g0;
+    }
+    gA = wave_sum63(gA);
+    gL = wave_sum63(gL);
+    gAl = wave_sum63(gAl);
+    gQ = wave_sum63(gQ);
+    gP = wave_sum63(gP);
+    if (lane == 63) {
       float* o = s_red + (i * 5) * 4 + wid;
       o[0 * 4] = gA;
       o[1 * 4] = gL;
@@ -197,21 +219,21 @@ __device__ __forceinline__ void adjoint_body(const FitDesc& P, int local_block, 
   }
 }
 
-__global__ __launch_bounds__(kFitThreads) void adjoint_kernel(const FitDesc* __restrict__ probs, int count,
+// One launch covers the blocks [base, base + gridDim.x) of the batch's block -> problem
+// map, all of one snapshot class.
+template <int SMAX, int KP>
+__global__ __launch_bounds__(kFitThreads) void adjoint_kernel(const FitDesc* __restrict__ probs,
+                                                              const int* __restrict__ block_prob, long long base,
                                                               int bump_step) {
   __shared__ FitWaveLds s_wave[kFitMaxWaves];
   __shared__ float s_red[(kFitMaxWaves * 5 + 1) * 4];
-  __shared__ int s_prob;
-  if (threadIdx.x == 0) s_prob = find_problem(probs, count, blockIdx.x, false);
-  __syncthreads();
-  const FitDesc P = probs[s_prob];
-  const int local_block = (int)(blockIdx.x - P.block_offset);
+  const long long gb = base + blockIdx.x;
+  const FitDesc P = probs[block_prob[gb]];
+  const int local_block = (int)(gb - P.block_offset);
   if (bump_step && local_block == 0 && threadIdx.x == 0) P.step[0] += 1;  // Adam step of this iteration
   for (int i = threadIdx.x; i < P.W; i += blockDim.x) stage_fit_wave(P, i, &s_wave[i]);
   __syncthreads();
-  if (P.S <= 8) adjoint_body<8>(P, local_block, s_wave, s_red);
-  else if (P.S <= 16) adjoint_body<16>(P, local_block, s_wave, s_red);
-  else adjoint_body<32>(P, local_block, s_wave, s_red);
+  adjoint_body<SMAX, KP>(P, local_block, s_wave, s_red);
   __syncthreads();
   // fixed-order sum over the 4 wavefronts, one partial per (row, block)
   const int rows = P.W * 5 + 1;
@@ -221,13 +243,12 @@ __global__ __launch_bounds__(kFitThreads) void adjoint_kernel(const FitDesc* __r
   }
 }
 
-__global__ __launch_bounds__(kFitThreads) void reduce_adam_kernel(const FitDesc* __restrict__ probs, int count,
-                                                                  FitAdam adam, int mode, float* __restrict__ grad_out,
+__global__ __launch_bounds__(kFitThreads) void reduce_adam_kernel(const FitDesc* __restrict__ probs,
+                                                                  const int* __restrict__ row_prob, FitAdam adam,
+                                                                  int mode, float* __restrict__ grad_out,
                                                                   float* __restrict__ loss_out) {
   __shared__ float s[kFitThreads];
-  __shared__ int s_prob;
-  if (threadIdx.x == 0) s_prob = find_problem(probs, count, blockIdx.x, true);
-  __syncthreads();
+  const int s_prob = row_prob[blockIdx.x];
   const FitDesc P = probs[s_prob];
   const int row = (int)(blockIdx.x - P.row_offset);
   const float* src = P.partials + (size_t)row * P.nblocks;
@@ -335,14 +356,27 @@ hipError_t launch_zero_words(uint32_t* p, size_t n, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_fit_iteration(const FitDesc* d_probs, int count, long long total_blocks, long long total_rows,
-                                const FitAdam& adam, int mode, float* grad_out, float* loss_out, hipStream_t st) {
-  hipLaunchKernelGGL(adjoint_kernel, dim3((unsigned)total_blocks), dim3(kFitThreads), 0, st, d_probs, count,
-                     mode == 1 ? 1 : 0);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(reduce_adam_kernel, dim3((unsigned)total_rows), dim3(kFitThreads), 0, st, d_probs, count, adam,
-                     mode, grad_out, loss_out);
+hipError_t launch_fit_iteration(const FitLaunch& L, const FitAdam& adam, int mode, float* grad_out, float* loss_out,
+                                hipStream_t st) {
+  const int bump = mode == 1 ? 1 : 0;
+  for (int c = 0; c < kFitClasses; ++c) {
+    if (L.class_blocks[c] == 0) continue;
+    const dim3 grid((unsigned)L.class_blocks[c]);
+    const long long base = L.class_begin[c];
+    if (c == 0)
+      hipLaunchKernelGGL((adjoint_kernel<8, kFitPointsPerThread[0]>), grid, dim3(kFitThreads), 0, st, L.probs,
+                         L.block_prob, base, bump);
+    else if (c == 1)
+      hipLaunchKernelGGL((adjoint_kernel<16, kFitPointsPerThread[1]>), grid, dim3(kFitThreads), 0, st, L.probs,
+                         L.block_prob, base, bump);
+    else
+      hipLaunchKernelGGL((adjoint_kernel<32, kFitPointsPerThread[2]>), grid, dim3(kFitThreads), 0, st, L.probs,
+                         L.block_prob, base, bump);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(reduce_adam_kernel, dim3((unsigned)L.total_rows), dim3(kFitThreads), 0, st, L.probs, L.row_prob,
+                     adam, mode, grad_out, loss_out);
   return hipGetLastError();
 }
 

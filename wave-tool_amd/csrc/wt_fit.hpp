@@ -10,6 +10,13 @@ namespace wt {
 
 constexpr int kFitMaxWaves = 128;
 constexpr int kFitMaxSnapshots = 32;
+constexpr int kFitThreads = 256;
+// Snapshot classes S <= 8, <= 16, <= 32 and the grid points each thread of their
+// adjoint kernel carries (its residuals, KP x SMAX x 3 floats, stay in registers).
+constexpr int kFitClasses = 3;
+constexpr int kFitClassMaxS[kFitClasses] = {8, 16, 32};
+constexpr int kFitPointsPerThread[kFitClasses] = {2, 1, 1};
+inline int fit_class(int S) { return S <= 8 ? 0 : (S <= 16 ? 1 : 2); }
 
 // One problem of a batch as the kernels see it. Device pointers are owned by the
 // wt_fit handle except params and target, which the caller owns.
@@ -22,8 +29,8 @@ struct FitDesc {
   float* partials;      // [W*5 + 1][nblocks]: per-block sums of r . dd/dp and of |r|^2 / 2
   float* loss;          // [1] loss of the last evaluation
   int* step;            // [1] Adam step counter
-  int n, S, W, nblocks;
-  long long block_offset;      // first block of this problem in the adjoint grid
+  int n, S, W, nblocks;         // nblocks: adjoint blocks of kFitThreads x KP points
+  long long block_offset;      // first block of this problem in the block -> problem map
   long long row_offset;        // first row of this problem in the reduce grid
   long long param_out_offset;  // first parameter of this problem in a concatenated gradient
 };
@@ -32,10 +39,21 @@ struct FitAdam {
   float lr, beta1, beta2, eps;
 };
 
+// Device tables of a batch: problems, block -> problem (blocks grouped by snapshot
+// class, class c at [class_begin[c], + class_blocks[c])), reduce row -> problem.
+struct FitLaunch {
+  const FitDesc* probs;
+  const int* block_prob;
+  const int* row_prob;
+  long long class_begin[kFitClasses];
+  long long class_blocks[kFitClasses];
+  long long total_rows;
+};
+
 // mode 0: gradient + loss only (grad_out receives the concatenated gradients);
 // mode 1: Adam step (the adjoint launch bumps each problem's step counter first).
-hipError_t launch_fit_iteration(const FitDesc* d_probs, int count, long long total_blocks, long long total_rows,
-                                const FitAdam& adam, int mode, float* grad_out, float* loss_out, hipStream_t st);
+hipError_t launch_fit_iteration(const FitLaunch& L, const FitAdam& adam, int mode, float* grad_out, float* loss_out,
+                                hipStream_t st);
 
 // Target of a problem from its own params (the hidden wave set) plus N(0, sigma) noise.
 hipError_t launch_synthesize(const FitDesc& P, float* out, float sigma, unsigned long long seed, hipStream_t st);

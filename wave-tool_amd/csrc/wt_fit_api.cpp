@@ -22,13 +22,13 @@ struct wt_fit {
   long long total_blocks = 0, total_rows = 0, total_params = 0;
   std::vector<wt::FitDesc> host;
   wt::FitDesc* d_desc = nullptr;
+  int* d_maps = nullptr;    // block -> problem, then row -> problem
   char* d_state = nullptr;  // moments, partials, loss, step for every problem
   size_t state_bytes = 0;
+  wt::FitLaunch launch{};
 };
 
 namespace {
-
-constexpr int kThreads = 256;
 
 wt_status hip_err(wt_ctx* ctx, hipError_t e, const char* where) {
   return wt::ctx_fail(ctx, WT_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
@@ -62,6 +62,7 @@ wt_status wt_fit_create(wt_ctx* ctx, const wt_fit_problem* problems, uint32_t co
   // state layout per problem: m[W*5], v[W*5], partials[(W*5+1)*nblocks], loss, step
   std::vector<size_t> off(count);
   size_t bytes = 0;
+  long long class_blocks[wt::kFitClasses] = {0, 0, 0};
   for (uint32_t i = 0; i < count; ++i) {
     std::string why;
     if (!problem_ok(problems[i], &why)) {
@@ -70,7 +71,9 @@ wt_status wt_fit_create(wt_ctx* ctx, const wt_fit_problem* problems, uint32_t co
     }
     const wt_fit_problem& p = problems[i];
     const long long V = (long long)p.grid_side * p.grid_side;
-    const int nblocks = (int)((V + kThreads - 1) / kThreads);
+    const int cls = wt::fit_class((int)p.snapshots);
+    const long long per_block = (long long)wt::kFitThreads * wt::kFitPointsPerThread[cls];
+    const int nblocks = (int)((V + per_block - 1) / per_block);
     const int np = (int)p.wave_count * 5;
     wt::FitDesc& d = f->host[i];
     d.spacing = p.spacing;
@@ -85,9 +88,10 @@ wt_status wt_fit_create(wt_ctx* ctx, const wt_fit_problem* problems, uint32_t co
     d.S = (int)p.snapshots;
     d.W = (int)p.wave_count;
     d.nblocks = nblocks;
-    d.block_offset = f->total_blocks;
+    d.block_offset = class_blocks[cls];  // made global below, once the class sizes are known
     d.row_offset = f->total_rows;
     d.param_out_offset = f->total_params;
+    class_blocks[cls] += nblocks;
     f->total_blocks += nblocks;
     f->total_rows += np + 1;
     f->total_params += np;
@@ -95,6 +99,15 @@ wt_status wt_fit_create(wt_ctx* ctx, const wt_fit_problem* problems, uint32_t co
     bytes += sizeof(float) * ((size_t)2 * np + (size_t)(np + 1) * nblocks + 1) + sizeof(int);
     bytes = (bytes + 255) & ~(size_t)255;
   }
+  // blocks grouped by snapshot class: class c occupies [class_begin[c], + class_blocks[c])
+  long long class_begin[wt::kFitClasses];
+  for (int c = 0, acc = 0; c < wt::kFitClasses; ++c) {
+    class_begin[c] = acc;
+    acc += (int)class_blocks[c];
+    f->launch.class_begin[c] = class_begin[c];
+    f->launch.class_blocks[c] = class_blocks[c];
+  }
+  for (uint32_t i = 0; i < count; ++i) f->host[i].block_offset += class_begin[wt::fit_class(f->host[i].S)];
   if (f->total_blocks >= (1ll << 31) || f->total_rows >= (1ll << 31)) {
     delete f;
     return wt::ctx_fail(ctx, WT_ERR_UNSUPPORTED, "batch too large for one launch");
@@ -115,17 +128,30 @@ wt_status wt_fit_create(wt_ctx* ctx, const wt_fit_problem* problems, uint32_t co
     d.loss = d.partials + (size_t)(np + 1) * d.nblocks;
     d.step = reinterpret_cast<int*>(d.loss + 1);
   }
+  std::vector<int> maps((size_t)f->total_blocks + (size_t)f->total_rows);
+  for (uint32_t i = 0; i < count; ++i) {
+    const wt::FitDesc& d = f->host[i];
+    for (int b = 0; b < d.nblocks; ++b) maps[(size_t)d.block_offset + b] = (int)i;
+    for (int r = 0; r <= d.W * 5; ++r) maps[(size_t)f->total_blocks + d.row_offset + r] = (int)i;
+  }
   e = hipMalloc(&f->d_desc, sizeof(wt::FitDesc) * count);
+  if (e == hipSuccess) e = hipMalloc(&f->d_maps, sizeof(int) * maps.size());
   if (e == hipSuccess) e = hipMemcpy(f->d_desc, f->host.data(), sizeof(wt::FitDesc) * count, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(f->d_maps, maps.data(), sizeof(int) * maps.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemset(f->d_state, 0, bytes);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) {
     wt_status s = hip_err(ctx, e, "wt_fit_create");
     if (f->d_desc) (void)hipFree(f->d_desc);
+    if (f->d_maps) (void)hipFree(f->d_maps);
     (void)hipFree(f->d_state);
     delete f;
     return s;
   }
+  f->launch.probs = f->d_desc;
+  f->launch.block_prob = f->d_maps;
+  f->launch.row_prob = f->d_maps + f->total_blocks;
+  f->launch.total_rows = f->total_rows;
   *out = f;
   return WT_OK;
 }
@@ -133,6 +159,7 @@ wt_status wt_fit_create(wt_ctx* ctx, const wt_fit_problem* problems, uint32_t co
 void wt_fit_destroy(wt_fit* f) {
   if (!f) return;
   if (f->d_desc) (void)hipFree(f->d_desc);
+  if (f->d_maps) (void)hipFree(f->d_maps);
   if (f->d_state) (void)hipFree(f->d_state);
   delete f;
 }
@@ -151,8 +178,7 @@ wt_status wt_fit_iterate(wt_fit* f, const wt_adam_config* adam, uint32_t iterati
     return wt::ctx_fail(f->ctx, WT_ERR_INVALID_ARG, "Adam settings out of range");
   const wt::FitAdam a{adam->lr, adam->beta1, adam->beta2, adam->eps};
   for (uint32_t it = 0; it < iterations; ++it) {
-    hipError_t e = wt::launch_fit_iteration(f->d_desc, f->count, f->total_blocks, f->total_rows, a, 1, nullptr,
-                                            d_losses, (hipStream_t)stream);
+    hipError_t e = wt::launch_fit_iteration(f->launch, a, 1, nullptr, d_losses, (hipStream_t)stream);
     if (e != hipSuccess) return hip_err(f->ctx, e, "fit iteration launch");
   }
   return WT_OK;
@@ -161,8 +187,7 @@ wt_status wt_fit_iterate(wt_fit* f, const wt_adam_config* adam, uint32_t iterati
 wt_status wt_fit_loss_grad(wt_fit* f, float* d_grad, float* d_losses, void* stream) {
   if (!f || !d_grad) return WT_ERR_INVALID_ARG;
   const wt::FitAdam a{0.0f, 0.0f, 0.0f, 0.0f};
-  hipError_t e = wt::launch_fit_iteration(f->d_desc, f->count, f->total_blocks, f->total_rows, a, 0, d_grad, d_losses,
-                                          (hipStream_t)stream);
+  hipError_t e = wt::launch_fit_iteration(f->launch, a, 0, d_grad, d_losses, (hipStream_t)stream);
   return e == hipSuccess ? WT_OK : hip_err(f->ctx, e, "fit gradient launch");
 }
 
