@@ -587,7 +587,8 @@ wt_status wt_water_displace_batched(wt_ctx* ctx, wt_batch* b, const wt_gerstner_
   a.bit = out->bitangent;
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || cus <= 0) cus = 256;
-  const long long resident = (long long)wt::batched_blocks_per_cu(b->shape, hm != nullptr) * cus;
+  const long long resident =
+      (long long)wt::batched_blocks_per_cu(b->shape, wt::tex_mode(a.texels, a.tex_fmt, a.tex_wrap)) * cus;
   const int grid = (int)std::min<long long>(resident, b->n_units);
   hipError_t e = wt::launch_forward_batched(a, b->shape, grid, st);
   if (e != hipSuccess) return hip_fail(ctx, e, "water_forward_batched_kernel launch");

@@ -57,6 +57,17 @@ __device__ __forceinline__ void stage_wave(const wt_gerstner_wave& g, double t, 
   o->cs[3] = qaw * Dz * Dz;
 }
 
+// Heightmap sampling mode, a compile-time parameter of the kernels: none, or texel
+// format x wrap mode (src/texture.cpp:54-72 sets the reference's to 8-bit, MIRRORED_REPEAT).
+enum TexMode : int { kTexNone = 0, kTexR32Repeat = 1, kTexR32Mirror = 2, kTexR8Repeat = 3, kTexR8Mirror = 4 };
+constexpr bool tex_r8(int m) { return m == kTexR8Repeat || m == kTexR8Mirror; }
+constexpr bool tex_mirror(int m) { return m == kTexR32Mirror || m == kTexR8Mirror; }
+__host__ __device__ inline int tex_mode(const void* texels, int fmt, int wrap) {
+  if (!texels) return kTexNone;
+  const bool r8 = fmt == WT_TEXEL_R8_UNORM, mirror = wrap == WT_WRAP_MIRRORED_REPEAT;
+  return r8 ? (mirror ? kTexR8Mirror : kTexR8Repeat) : (mirror ? kTexR32Mirror : kTexR32Repeat);
+}
+
 // Bilinear heightmap tap with GL texel-centre and wrap rules, split in two so a lane can
 // issue the texel loads of all its vertices before it needs any of them.
 // heightmap_fetch: (s, t) are texture coordinates already reduced to one wrap period
@@ -66,47 +77,48 @@ struct HmTap {
   float t00, t10, t01, t11, a, b;
 };
 
-__device__ __forceinline__ HmTap heightmap_fetch(const void* __restrict__ texels, int W, int H, int fmt, int wrap_mode,
-                                                 float s, float t) {
-  const bool repeat = wrap_mode == WT_WRAP_REPEAT;
-  const int period_w = repeat ? W : 2 * W;
-  const int period_h = repeat ? H : 2 * H;
+template <int M>
+__device__ __forceinline__ int wrap_index(int i, int n) {
+  // i in [-1, period]: one period of REPEAT (n) or MIRRORED_REPEAT (2n) folded onto [0, n)
+  const int period = tex_mirror(M) ? 2 * n : n;
+  i = i < 0 ? i + period : (i >= period ? i - period : i);
+  if (tex_mirror(M)) i = i < n ? i : (2 * n - 1 - i);
+  return i;
+}
+
+template <int M>
+__device__ __forceinline__ HmTap heightmap_fetch(const void* __restrict__ texels, int W, int H, float s, float t) {
+  const int period_w = tex_mirror(M) ? 2 * W : W;
+  const int period_h = tex_mirror(M) ? 2 * H : H;
   const float u = s * (float)W - 0.5f, v = t * (float)H - 0.5f;
   const float fu = floorf(u), fv = floorf(v);
   HmTap h;
   h.a = u - fu;
   h.b = v - fv;
   // fmaxf/fminf return the non-NaN operand, so a NaN coordinate lands on texel -1
-  int i0 = (int)fminf(fmaxf(fu, -1.0f), (float)(period_w - 1));
-  int j0 = (int)fminf(fmaxf(fv, -1.0f), (float)(period_h - 1));
-  int i1 = i0 + 1, j1 = j0 + 1;
-  auto wrap = [repeat](int i, int n, int period) {
-    i = i < 0 ? i + period : (i >= period ? i - period : i);
-    if (!repeat) i = i < n ? i : (2 * n - 1 - i);
-    return i;
-  };
-  i0 = wrap(i0, W, period_w);
-  i1 = wrap(i1, W, period_w);
-  j0 = wrap(j0, H, period_h);
-  j1 = wrap(j1, H, period_h);
-  if (fmt == WT_TEXEL_R8_UNORM) {
+  const int i0 = (int)fminf(fmaxf(fu, -1.0f), (float)(period_w - 1));
+  const int j0 = (int)fminf(fmaxf(fv, -1.0f), (float)(period_h - 1));
+  const int c0 = wrap_index<M>(i0, W), c1 = wrap_index<M>(i0 + 1, W);
+  const int r0 = wrap_index<M>(j0, H) * W, r1 = wrap_index<M>(j0 + 1, H) * W;
+  if (tex_r8(M)) {
     const uint8_t* p = (const uint8_t*)texels;
-    h.t00 = (float)p[j0 * W + i0];
-    h.t10 = (float)p[j0 * W + i1];
-    h.t01 = (float)p[j1 * W + i0];
-    h.t11 = (float)p[j1 * W + i1];
+    h.t00 = (float)p[r0 + c0];
+    h.t10 = (float)p[r0 + c1];
+    h.t01 = (float)p[r1 + c0];
+    h.t11 = (float)p[r1 + c1];
   } else {
     const float* p = (const float*)texels;
-    h.t00 = p[j0 * W + i0];
-    h.t10 = p[j0 * W + i1];
-    h.t01 = p[j1 * W + i0];
-    h.t11 = p[j1 * W + i1];
+    h.t00 = p[r0 + c0];
+    h.t10 = p[r0 + c1];
+    h.t01 = p[r1 + c0];
+    h.t11 = p[r1 + c1];
   }
   return h;
 }
 
-__device__ __forceinline__ float heightmap_eval(HmTap h, int W, int H, int fmt, float* dds, float* ddt) {
-  if (fmt == WT_TEXEL_R8_UNORM) {  // unorm8: c / 255
+template <int M>
+__device__ __forceinline__ float heightmap_eval(HmTap h, int W, int H, float* dds, float* ddt) {
+  if (tex_r8(M)) {  // unorm8: c / 255
     const float k = 1.0f / 255.0f;
     h.t00 *= k; h.t10 *= k; h.t01 *= k; h.t11 *= k;
   }
@@ -138,30 +150,30 @@ struct VertexPending {
 };
 
 // acc: X, Z, Cx, Cz (cos sums) and Y, Sxx, Sxz, Szz (sin sums).
-template <int kHm>
+template <int M>
 __device__ __forceinline__ VertexPending vertex_fetch(const EpilogueArgs& e, double x, double z, const float acc[8]) {
   VertexPending v;
   const double pxd = x + (double)acc[0], pzd = z + (double)acc[1];
   v.px = (float)pxd;
   v.pz = (float)pzd;
-  if (kHm) {
+  if constexpr (M != kTexNone) {
     // water-grid.vert:103-110: sample at s_h * (x, -z) of the displaced point. The
     // texture coordinate is formed from the FP64 displaced point and reduced to one
     // wrap period in FP64, so its FP32 rounding is relative to a number below 2.
     double sd = fma((double)e.s_h, pxd, e.hm_off_s), td = fma(-(double)e.s_h, pzd, e.hm_off_t);
-    if (e.tex_wrap == WT_WRAP_REPEAT) {
-      sd -= floor(sd);
-      td -= floor(td);
-    } else {
+    if (tex_mirror(M)) {
       sd -= 2.0 * floor(0.5 * sd);
       td -= 2.0 * floor(0.5 * td);
+    } else {
+      sd -= floor(sd);
+      td -= floor(td);
     }
-    v.tap = heightmap_fetch(e.texels, e.tex_w, e.tex_h, e.tex_fmt, e.tex_wrap, (float)sd, (float)td);
+    v.tap = heightmap_fetch<M>(e.texels, e.tex_w, e.tex_h, (float)sd, (float)td);
   }
   return v;
 }
 
-template <int kHm>
+template <int M>
 __device__ __forceinline__ void vertex_finish(const EpilogueArgs& e, const VertexPending& v, const float acc[8],
                                               size_t vid) {
   const float Cx = acc[2], Cz = acc[3];
@@ -172,9 +184,9 @@ __device__ __forceinline__ void vertex_finish(const EpilogueArgs& e, const Verte
   // reference differences neighbours, water-grid.vert:164-187)
   float tx = 1.0f - Sxx, ty = Cx, tz = -Sxz;
   float bx = -Sxz, by = Cz, bz = 1.0f - Szz;
-  if (kHm) {
+  if constexpr (M != kTexNone) {
     float dds, ddt;
-    const float r = heightmap_eval(v.tap, e.tex_w, e.tex_h, e.tex_fmt, &dds, &ddt);
+    const float r = heightmap_eval<M>(v.tap, e.tex_w, e.tex_h, &dds, &ddt);
     py += e.s_d * (2.0f * (r - 0.5f));  // water-grid.vert:108-110
     const float k = 2.0f * e.s_d * e.s_h;
     const float hpx = k * dds, hpz = -k * ddt;  // d(bump)/dPx, d(bump)/dPz

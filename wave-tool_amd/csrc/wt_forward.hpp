@@ -1,0 +1,446 @@
+// Forward kernels (templates): the water vertex program (assets/shaders/water-grid.vert:114-193)
+// over a gridLength^2 projected grid, with analytic normal / tangent / bitangent.
+//
+// Work decomposition (gfx950, wave64):
+//  * A "segment" is LR = 64/LC grid rows x (LC*K) columns. Lane l sits on row l/LC and
+//    column c0 + (l % LC); its k-th vertex is LC*k columns further. Consecutive lanes
+//    hold consecutive columns, so every store instruction writes a contiguous run of
+//    float3 (coalesced).
+//  * S wavefronts share one segment: wavefront s of the group sums waves s, s+S, ...
+//    The S partial sums are exchanged through LDS at the end and each wavefront runs
+//    the epilogue for K/S of the K vertices. S = 2 doubles the number of resident
+//    wavefronts for the same per-lane work, which is what lets gfx950 dual-issue VALU
+//    instructions (two waves per SIMD per quad-cycle).
+//  * Along a row the grid point is affine in the column (bilinear corners), so each
+//    wave's phase advances by a constant angle per LC columns. A lane seeds
+//    (cos, sin) once per wave from an FP64-reduced phase (native v_sin / v_cos on an
+//    argument in [0, 1] revolution) and steps K vertices with an FP32 rotation (4 VALU
+//    ops per vertex-wave instead of two transcendentals). The rotation per (row, wave)
+//    is computed in FP64 and rounded once.
+//  * Per-wave constants are staged in LDS per chunk of kChunk waves (WaveLds), the
+//    per-row rotations in a per-wavefront LDS table; the next wave's constants are
+//    loaded while the current wave is summed.
+//  * 8 running sums per vertex: x/z displacement, y, and the 5 distinct entries of the
+//    analytic Jacobian, all VALU FMAs with VGPR operands (no SGPR operand, which would
+//    block dual issue).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "wt_device.hpp"
+#include "wt_kernels.hpp"
+
+namespace wt {
+
+constexpr int kChunk = 64;      // waves staged per LDS pass
+constexpr int kWavesPerWG = 4;  // 4 wavefronts of 64 = 256 threads
+
+// FP64 row geometry: x(col) = xa + col * xs, z(col) = za + col * zs.
+struct RowGeom {
+  double xa, xs, za, zs;
+};
+__device__ __forceinline__ RowGeom row_geom(const FwdParams& p, int row) {
+  const double v = (double)row * p.inv_nm1;
+  RowGeom g;
+  g.xa = fma(v, p.dv_x, p.bl_x);
+  g.za = fma(v, p.dv_z, p.bl_z);
+  g.xs = fma(v, p.duv_x, p.du_x) * p.inv_nm1;
+  g.zs = fma(v, p.duv_z, p.du_z) * p.inv_nm1;
+  return g;
+}
+
+// Stage waves [w0, w0 + nw) of the frame into LDS (every thread of the block calls).
+__device__ __forceinline__ void stage_chunk(const FwdParams& p, int w0, int nw, WaveLds* s_wave) {
+  for (int t = threadIdx.x; t < nw; t += blockDim.x) stage_wave(p.waves[w0 + t], p.time, &s_wave[t]);
+}
+
+// Per (row of a segment, wave of a wavefront) entry of the step table.
+//  rot:    cos, sin, -sin of the phase advance over LC columns, and 2 cos of it;
+//  seed:   phase of the segment's first column (revolutions, [0, 1)), the per-column
+//          advance reduced to [0, 1) and split into hi (a multiple of 2^-18, so that
+//          lc * hi is exact in FP32 for lc < 64) and lo;
+//  cc, cs: the wave's cos / sin coefficients (WaveLds::cc, ::cs).
+struct StepEntry {
+  float4 rot;
+  float4 seed;
+  float4 cc;
+  float4 cs;
+};
+
+// The three-term (Chebyshev) recurrence x[j+1] = 2 cos(d) x[j] - x[j-1] generates the
+// cos / sin sequence with one FMA per value instead of the rotation's two, but an FP32
+// rounding e of 2 cos(d) shifts the step by e / (2 |sin d|) and local rounding errors
+// grow by up to 1 / |sin d|; with |sin d| >= 0.3 and K <= 16 steps both stay below
+// 2e-6 rad / 6e-6 relative. A wave takes the recurrence for a segment when every row of
+// the segment satisfies this, i.e. when the step d = 2 pi f has f mod 1/2 in
+// [kChebMinRev, 1/2 - kChebMinRev], kChebMinRev = asin(0.3) / (2 pi).
+constexpr double kChebMinRev = 0.048493;
+
+// Build the step table of the waves this wavefront sums (i = sid, sid + S, ...) for the
+// LR rows of the segment starting at column col_base, in FP64, rounded once. Waves that
+// take the three-term recurrence come first; returns their number (wave-uniform).
+template <int LC, int LR, int S>
+__device__ __forceinline__ int build_table(const FwdParams& p, int row_base, int col_base, int nw, int sid,
+                                           const WaveLds* s_wave, StepEntry* tab, int lane) {
+  constexpr int kMine = kChunk / S;
+  const int nmine = (nw - sid + S - 1) / S;
+  // which waves qualify for the recurrence on every row of the segment
+  bool cheb = lane < nmine;
+  if (cheb) {
+    const WaveLds& w = s_wave[sid + lane * S];
+#pragma unroll
+    for (int r = 0; r < LR; ++r) {
+      int row = row_base + r;
+      row = row < p.N ? row : p.N - 1;
+      const RowGeom g = row_geom(p, row);
+      const double st = fma(w.kx, g.xs, w.kz * g.zs) * (double)LC;
+      const double h = 2.0 * (st - floor(st));
+      const double f = 0.5 * (h - floor(h));  // step mod 1/2, in revolutions
+      cheb = cheb && f >= kChebMinRev && f <= 0.5 - kChebMinRev;
+    }
+  }
+  const uint64_t valid = nmine >= 64 ? ~0ull : ((1ull << nmine) - 1ull);
+  const uint64_t cmask = __ballot(cheb) & valid;
+  const int ncheb = __popcll(cmask);
+  for (int e = lane; e < LR * nmine; e += 64) {
+    const int r = e / nmine, m = e - r * nmine, i = sid + m * S;
+    const uint64_t mb = (1ull << m) - 1ull;
+    const int pos = ((cmask >> m) & 1ull) ? __popcll(cmask & mb) : ncheb + __popcll(~cmask & valid & mb);
+    int row = row_base + r;
+    row = row < p.N ? row : p.N - 1;
+    const RowGeom g = row_geom(p, row);
+    const WaveLds& w = s_wave[i];
+    const double q = fma(w.kx, g.xs, w.kz * g.zs);  // revolutions per column
+    const double qf = q - floor(q);
+    const double step = qf * (double)LC;
+    float cr, sr;
+    sincos_rev(step - floor(step), &cr, &sr);
+    const double ph = fma((double)col_base, q, fma(w.kx, g.xa, fma(w.kz, g.za, w.c0)));
+    const double qh = floor(qf * 262144.0) * (1.0 / 262144.0);
+    StepEntry t;
+    t.rot = make_float4(cr, sr, -sr, 2.0f * cr);
+    t.seed = make_float4((float)(ph - floor(ph)), (float)qh, (float)(qf - qh), 0.0f);
+    t.cc = make_float4(w.cc[0], w.cc[1], w.cc[2], w.cc[3]);
+    t.cs = make_float4(w.cs[0], w.cs[1], w.cs[2], w.cs[3]);
+    tab[r * kMine + pos] = t;
+  }
+  return ncheb;
+}
+
+// cos / sin of the phase of column col_base + lc from a seed entry: lc * hi is exact,
+// so only quantities below 3 revolutions are rounded.
+__device__ __forceinline__ void seed_trig(const float4& sd, float lcf, float* c, float* s) {
+  const float r = fmaf(lcf, sd.z, __builtin_amdgcn_fractf(lcf * sd.y) + sd.x);
+  *c = __builtin_amdgcn_cosf(r);
+  *s = __builtin_amdgcn_sinf(r);
+}
+
+// Partial sums of K consecutive vertices (LC columns apart) for one wave, steps
+// [J0, J1), rotation form: (c, s) holds step J0 on entry and step J1 on exit (J1 - 1
+// when J1 == K).
+template <int K>
+__device__ __forceinline__ void accumulate(float (&a)[8], float c, float s, const float4& cc, const float4& cs) {
+  a[0] = fmaf(cc.x, c, a[0]);
+  a[1] = fmaf(cc.y, c, a[1]);
+  a[2] = fmaf(cc.z, c, a[2]);
+  a[3] = fmaf(cc.w, c, a[3]);
+  a[4] = fmaf(cs.x, s, a[4]);
+  a[5] = fmaf(cs.y, s, a[5]);
+  a[6] = fmaf(cs.z, s, a[6]);
+  a[7] = fmaf(cs.w, s, a[7]);
+}
+
+template <int K, int J0, int J1>
+__device__ __forceinline__ void steps_rotation(float (&acc)[K][8], float& c, float& s, const float4& cc,
+                                               const float4& cs, const float4& rot) {
+#pragma unroll
+  for (int j = J0; j < J1; ++j) {
+    accumulate<K>(acc[j], c, s, cc, cs);
+    if (j + 1 < K) {
+      // (c, s) <- (c cr - s sr, s cr + c sr); -sr is staged so every op is a plain FMA
+      const float cn = fmaf(s, rot.z, c * rot.x);
+      s = fmaf(c, rot.y, s * rot.x);
+      c = cn;
+    }
+    // keep the recurrence interleaved with the sums: hoisting all K steps of it ahead
+    // of the FMAs would hold 2K extra live values
+    if ((j & 1) && j + 1 < J1) __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// The same steps with the three-term recurrence; step 0 -> 1 is a rotation, which also
+// sets the previous values (cm, sm).
+template <int K, int J0, int J1>
+__device__ __forceinline__ void steps_chebyshev(float (&acc)[K][8], float& c, float& s, float& cm, float& sm,
+                                                const float4& cc, const float4& cs, const float4& rot) {
+#pragma unroll
+  for (int j = J0; j < J1; ++j) {
+    accumulate<K>(acc[j], c, s, cc, cs);
+    if (j + 1 < K) {
+      float cn, sn;
+      if (j == 0) {
+        cn = fmaf(s, rot.z, c * rot.x);
+        sn = fmaf(c, rot.y, s * rot.x);
+      } else {
+        cn = fmaf(rot.w, c, -cm);
+        sn = fmaf(rot.w, s, -sm);
+      }
+      cm = c;
+      sm = s;
+      c = cn;
+      s = sn;
+    }
+    if ((j & 1) && j + 1 < J1) __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// Epilogue of vertices [JB, JB + KS) of a lane: all texel loads first, then the per-vertex
+// math and stores (JB is a compile-time constant so the sums stay in registers).
+template <int K, int LC, int KS, int JB, int kTex>
+__device__ __forceinline__ void epilogue_steps(const FwdParams& p, const RowGeom& g, const float (&acc)[K][8],
+                                               int col0, int row, bool active) {
+  VertexPending pend[KS];
+#pragma unroll
+  for (int u = 0; u < KS; ++u) {
+    const int col = col0 + (JB + u) * LC;
+    const double x = fma((double)col, g.xs, g.xa), z = fma((double)col, g.zs, g.za);
+    pend[u] = vertex_fetch<kTex>(p.ep, x, z, acc[JB + u]);
+  }
+#pragma unroll
+  for (int u = 0; u < KS; ++u) {
+    const int col = col0 + (JB + u) * LC;
+    if (active && row < p.N && col < p.N)
+      vertex_finish<kTex>(p.ep, pend[u], acc[JB + u], (size_t)row * (size_t)p.N + (size_t)col);
+  }
+}
+
+// LDS of one workgroup of the forward kernels. The wave constants and step tables are
+// dead once the main loop ends, so the S = 2 exchange of partial sums reuses their space.
+template <int K, int LC, int S>
+struct SegShared {
+  static constexpr int LR = 64 / LC;
+  static constexpr int KS = K / S;
+  static constexpr int kSegPerWG = kWavesPerWG / S;
+  static constexpr int kTab = LR * (kChunk / S);  // step-table entries per wavefront
+  static constexpr size_t kLoopBytes = sizeof(WaveLds) * kChunk + sizeof(StepEntry) * kWavesPerWG * kTab;
+  // partial sums handed to the other wavefront of the pair: [pair][sender][K-KS][8][64]
+  static constexpr size_t kXchgFloats = (size_t)(K - KS) * 8 * 64;
+  static constexpr size_t kXchgBytes = S > 1 ? sizeof(float) * kSegPerWG * S * kXchgFloats : 0;
+  alignas(16) unsigned char raw[kLoopBytes > kXchgBytes ? kLoopBytes : kXchgBytes];
+  __device__ WaveLds* wave() { return reinterpret_cast<WaveLds*>(raw); }
+  __device__ StepEntry* tab(int wid) {
+    return reinterpret_cast<StepEntry*>(raw + sizeof(WaveLds) * kChunk) + wid * kTab;
+  }
+  __device__ float* xchg(int grp, int sender) { return reinterpret_cast<float*>(raw) + (grp * S + sender) * kXchgFloats; }
+};
+static_assert(sizeof(WaveLds) * kChunk % 16 == 0, "step table alignment");
+
+// One workgroup's segments of one grid: the whole forward program for the vertices of
+// segment group `seg_group` (kSegPerWG segments). Every thread of the block calls it.
+template <int K, int LC, int S, int kTex>
+__device__ __forceinline__ void forward_segments(const FwdParams& p, long long seg_group, SegShared<K, LC, S>& sh) {
+  constexpr int LR = 64 / LC;
+  constexpr int KS = K / S;                       // vertices per lane in the epilogue
+  constexpr int kSegPerWG = kWavesPerWG / S;
+  WaveLds* s_wave = sh.wave();
+
+  const int lane = threadIdx.x & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+  const int grp = wid / S, sid = wid - grp * S;
+  const long long seg_global = seg_group * kSegPerWG + grp;
+  const int segs_per_rg = (p.N + LC * K - 1) / (LC * K);
+  const int n_rg = (p.N + LR - 1) / LR;
+  const bool active = seg_global < (long long)segs_per_rg * n_rg;
+  const int seg_id = active ? (int)seg_global : 0;
+  const int rg = seg_id / segs_per_rg, seg = seg_id - rg * segs_per_rg;
+  const int row_base = rg * LR;
+  const int lr = lane / LC, lc = lane - lr * LC;
+  const int row = row_base + lr;
+  const int col0 = seg * LC * K + lc;
+  const int col_base = seg * LC * K;
+  const float lcf = (float)lc;
+
+  float acc[K][8];
+#pragma unroll
+  for (int j = 0; j < K; ++j)
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc[j][q] = 0.0f;
+
+  for (int w0 = 0; w0 < p.W; w0 += kChunk) {
+    const int nw = p.W - w0 < kChunk ? p.W - w0 : kChunk;
+    __syncthreads();
+    stage_chunk(p, w0, nw, s_wave);
+    __syncthreads();
+    const int ncheb = build_table<LC, LR, S>(p, row_base, col_base, nw, sid, s_wave, sh.tab(wid), lane);
+    __syncthreads();
+    if (sid >= nw) continue;
+    const StepEntry* tab = sh.tab(wid) + lr * (kChunk / S);
+    // Software pipeline over this wavefront's table entries (three-term recurrence
+    // first, then rotation): the seed of entry k + 1 is formed between the two halves
+    // of entry k's steps (scheduling barriers keep the compiler from hoisting its
+    // dependent chain back to the top of the iteration).
+    const int nmine = (nw - sid + S - 1) / S;
+    float c, s, cm = 0.0f, sm = 0.0f;
+    seed_trig(tab[0].seed, lcf, &c, &s);
+    int k = 0;
+    for (; k < ncheb; ++k) {
+      const StepEntry& t = tab[k];
+      const float4 rot = t.rot, cc = t.cc, cs = t.cs;
+      const float4 sd = tab[k + 1 < nmine ? k + 1 : k].seed;
+      float cn, sn;
+      steps_chebyshev<K, 0, K / 2>(acc, c, s, cm, sm, cc, cs, rot);
+      __builtin_amdgcn_sched_barrier(0);
+      seed_trig(sd, lcf, &cn, &sn);
+      __builtin_amdgcn_sched_barrier(0);
+      steps_chebyshev<K, K / 2, K>(acc, c, s, cm, sm, cc, cs, rot);
+      c = cn;
+      s = sn;
+    }
+    for (; k < nmine; ++k) {
+      const StepEntry& t = tab[k];
+      const float4 rot = t.rot, cc = t.cc, cs = t.cs;
+      const float4 sd = tab[k + 1 < nmine ? k + 1 : k].seed;
+      float cn, sn;
+      steps_rotation<K, 0, K / 2>(acc, c, s, cc, cs, rot);
+      __builtin_amdgcn_sched_barrier(0);
+      seed_trig(sd, lcf, &cn, &sn);
+      __builtin_amdgcn_sched_barrier(0);
+      steps_rotation<K, K / 2, K>(acc, c, s, cc, cs, rot);
+      c = cn;
+      s = sn;
+    }
+  }
+  const RowGeom g = row_geom(p, row < p.N ? row : p.N - 1);
+
+  if (S > 1) {
+    // S = 2: wavefront 0 keeps vertices [0, K/2), wavefront 1 keeps [K/2, K); each hands
+    // the other half of its partial sums over through LDS and adds the half it receives
+    __syncthreads();
+    float* mine = sh.xchg(grp, sid);
+#pragma unroll
+    for (int j = 0; j < KS; ++j) {
+      const int jo = sid == 0 ? j + KS : j;  // the step owned by the other wavefront
+#pragma unroll
+      for (int q = 0; q < 8; ++q) mine[(j * 8 + q) * 64 + lane] = acc[jo][q];
+    }
+    __syncthreads();
+    const float* theirs = sh.xchg(grp, sid ^ 1);
+#pragma unroll
+    for (int j = 0; j < KS; ++j) {
+      const int jm = sid == 0 ? j : j + KS;  // my step
+#pragma unroll
+      for (int q = 0; q < 8; ++q) acc[jm][q] += theirs[(j * 8 + q) * 64 + lane];
+    }
+  }
+
+  // epilogue for the KS vertices this wavefront owns (steps [sid*KS, sid*KS + KS))
+  if (S == 1 || sid == 0)
+    epilogue_steps<K, LC, KS, 0, kTex>(p, g, acc, col0, row, active);
+  else
+    epilogue_steps<K, LC, KS, (S > 1 ? KS : 0), kTex>(p, g, acc, col0, row, active);
+}
+
+template <int K, int LC, int S, int kTex>
+__global__ __launch_bounds__(256) void water_forward_kernel(FwdParams p) {
+  __shared__ SegShared<K, LC, S> sh;
+  forward_segments<K, LC, S, kTex>(p, (long long)blockIdx.x, sh);
+}
+
+// ------------------------------------------------------------------ batched
+// Forward parameters of one patch of a batch (corner algebra done on the host).
+__device__ __forceinline__ FwdParams patch_params(const BatchArgs& a, const PatchDesc& pd) {
+  FwdParams p;
+  p.bl_x = pd.bl_x; p.bl_z = pd.bl_z;
+  p.du_x = pd.du_x; p.du_z = pd.du_z;
+  p.dv_x = pd.dv_x; p.dv_z = pd.dv_z;
+  p.duv_x = pd.duv_x; p.duv_z = pd.duv_z;
+  p.inv_nm1 = pd.inv_nm1;
+  p.time = pd.time + a.time_offset;
+  p.N = pd.N;
+  p.W = pd.W;
+  p.waves = a.waves + pd.wave_offset;
+  EpilogueArgs& e = p.ep;
+  e.cam[0] = pd.cam[0]; e.cam[1] = pd.cam[1]; e.cam[2] = pd.cam[2];
+  e.s_h = pd.s_h;
+  e.s_d = pd.s_d;
+  e.bounce = pd.bounce;
+  e.texels = a.texels;
+  e.tex_w = a.tex_w; e.tex_h = a.tex_h; e.tex_fmt = a.tex_fmt; e.tex_wrap = a.tex_wrap;
+  // texture-space scroll offset -s_h v t reduced to one wrap period in FP64
+  const double period = a.tex_wrap == WT_WRAP_REPEAT ? 1.0 : 2.0;
+  const double os = -(double)pd.s_h * (double)pd.scroll_x * p.time, ot = (double)pd.s_h * (double)pd.scroll_z * p.time;
+  e.hm_off_s = os - period * floor(os / period);
+  e.hm_off_t = ot - period * floor(ot / period);
+  const size_t base = 3 * (size_t)pd.vertex_offset;
+  e.pos = a.pos + base;
+  e.nrm = a.nrm + base;
+  e.tan = a.tan ? a.tan + base : nullptr;
+  e.bit = a.bit ? a.bit + base : nullptr;
+  return p;
+}
+
+// Persistent kernel over a batch of patches: each workgroup takes the next unit (one
+// workgroup's segments of one patch; units are sorted by cost, largest first) from a
+// device counter until the list is exhausted. The last workgroup to finish resets the
+// counters, so the launch can be replayed from a captured graph without a memset node.
+template <int K, int LC, int S, int kTex>
+__global__ __launch_bounds__(256) void water_forward_batched_kernel(BatchArgs a) {
+  __shared__ SegShared<K, LC, S> sh;
+  __shared__ int s_unit;
+  for (;;) {
+    if (threadIdx.x == 0) s_unit = (int)atomicAdd(&a.counters[0], 1u);
+    __syncthreads();
+    const int u = __builtin_amdgcn_readfirstlane(s_unit);  // wave-uniform: patch fields load as scalars
+    __syncthreads();  // every thread has read s_unit before it is overwritten
+    if (u >= a.n_units) break;
+    const BatchUnit unit = a.units[u];
+    const FwdParams p = patch_params(a, a.patches[unit.patch]);
+    forward_segments<K, LC, S, kTex>(p, (long long)unit.group, sh);
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const unsigned done = atomicAdd(&a.counters[1], 1u);
+    if (done == gridDim.x - 1) {
+      atomicExch(&a.counters[0], 0u);
+      atomicExch(&a.counters[1], 0u);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ launch helpers
+template <int K, int LC, int S, int kTex>
+hipError_t launch_t(const FwdParams& p, hipStream_t st) {
+  constexpr int LR = 64 / LC;
+  constexpr int kSegPerWG = kWavesPerWG / S;
+  const long long segs = (long long)((p.N + LC * K - 1) / (LC * K)) * ((p.N + LR - 1) / LR);
+  const long long blocks = (segs + kSegPerWG - 1) / kSegPerWG;
+  hipLaunchKernelGGL((water_forward_kernel<K, LC, S, kTex>), dim3((unsigned)blocks), dim3(256), 0, st, p);
+  return hipGetLastError();
+}
+
+// Every launch shape of the plain kernel for one texture mode (instantiated once per
+// mode, each in its own translation unit so the builds run in parallel).
+template <int kTex>
+hipError_t launch_forward_mode(const FwdParams& p, const LaunchShape& s, hipStream_t st) {
+  auto by_split = [&](auto k_tag, auto lc_tag) -> hipError_t {
+    constexpr int K = decltype(k_tag)::value, LC = decltype(lc_tag)::value;
+    return s.split == 2 ? launch_t<K, LC, 2, kTex>(p, st) : launch_t<K, LC, 1, kTex>(p, st);
+  };
+  auto by_lc = [&](auto k_tag) -> hipError_t {
+    switch (s.lanes_per_row) {
+      case 64: return by_split(k_tag, std::integral_constant<int, 64>{});
+      case 32: return by_split(k_tag, std::integral_constant<int, 32>{});
+      case 16: return by_split(k_tag, std::integral_constant<int, 16>{});
+      default: return by_split(k_tag, std::integral_constant<int, 8>{});
+    }
+  };
+  switch (s.steps) {
+    case 4: return by_lc(std::integral_constant<int, 4>{});
+    case 8: return by_lc(std::integral_constant<int, 8>{});
+    default: return by_lc(std::integral_constant<int, 16>{});
+  }
+}
+
+}  // namespace wt

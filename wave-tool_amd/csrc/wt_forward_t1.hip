@@ -1,0 +1,8 @@
+// Plain forward kernel, every launch shape, texture mode kTexR32Repeat.
+#include "wt_forward.hpp"
+
+namespace wt {
+hipError_t launch_forward_t1(const FwdParams& p, const LaunchShape& s, hipStream_t st) {
+  return launch_forward_mode<kTexR32Repeat>(p, s, st);
+}
+}  // namespace wt

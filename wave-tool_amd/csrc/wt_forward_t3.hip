@@ -1,0 +1,8 @@
+// Plain forward kernel, every launch shape, texture mode kTexR8Repeat.
+#include "wt_forward.hpp"
+
+namespace wt {
+hipError_t launch_forward_t3(const FwdParams& p, const LaunchShape& s, hipStream_t st) {
+  return launch_forward_mode<kTexR8Repeat>(p, s, st);
+}
+}  // namespace wt

@@ -67,6 +67,6 @@ long long forward_segment_groups(int N, const LaunchShape& s);
 // Launches the persistent batched kernel with `grid` workgroups.
 hipError_t launch_forward_batched(const BatchArgs& a, const LaunchShape& s, int grid, hipStream_t st);
 // Workgroups of the batched kernel resident per CU for this shape.
-int batched_blocks_per_cu(const LaunchShape& s, bool heightmap);
+int batched_blocks_per_cu(const LaunchShape& s, int tex_mode);
 
 }  // namespace wt
