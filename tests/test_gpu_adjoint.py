@@ -37,7 +37,8 @@ def oracle_grad(prob, tgt, params):
     return orc.fit_loss_grad(fd, params, nthreads=NTHREADS)
 
 
-@pytest.mark.parametrize("n,S,W", [(16, 2, 3), (37, 5, 8), (64, 16, 32), (130, 32, 7), (1, 1, 1)])
+@pytest.mark.parametrize("n,S,W", [(16, 2, 3), (37, 5, 8), (20, 8, 5), (19, 17, 4), (64, 16, 32), (130, 32, 7),
+                                   (1, 1, 1)])
 def test_gradient_matches_fp64_adjoint(torch_cuda, wtctx, n, S, W):
     torch = torch_cuda
     say("adjoint n=%d S=%d W=%d" % (n, S, W))

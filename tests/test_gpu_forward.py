@@ -169,6 +169,28 @@ def test_r8_mirrored_heightmap_and_bounce(torch_cuda, wtctx):
     assert res["pos_max"] <= POS_TOL and res["nrm_max"] <= NRM_TOL
 
 
+@pytest.mark.parametrize("fmt,wrap", [(wt.WT_TEXEL_R32F, wt.WT_WRAP_REPEAT), (wt.WT_TEXEL_R32F, wt.WT_WRAP_MIRRORED_REPEAT),
+                                      (wt.WT_TEXEL_R8_UNORM, wt.WT_WRAP_REPEAT),
+                                      (wt.WT_TEXEL_R8_UNORM, wt.WT_WRAP_MIRRORED_REPEAT)])
+def test_texture_modes(torch_cuda, wtctx, fmt, wrap):
+    """Each texel format x wrap mode (a separate kernel instantiation) on a non-square map
+    with scroll, against the oracle."""
+    say("texture mode fmt=%d wrap=%d" % (fmt, wrap))
+    cam = scenes.camera_tuple(height=5.0, pitch_deg=-12.0, yaw_deg=75.0)
+    corners = wt.projector_corners(cam, 2.5)
+    waves = scenes.gen_waves(17, 24)
+    rng = np.random.default_rng(3)
+    if fmt == wt.WT_TEXEL_R8_UNORM:
+        texels = rng.integers(0, 256, size=(40, 56), dtype=np.uint8)
+    else:
+        texels = scenes.value_noise(9, 64)[:40, :56].copy()
+    cfg = dict(N=257, sample_scale=0.03, displacement_scale=0.2, scroll=(0.4, -0.25))
+    res = check_parity(torch_cuda, wtctx, cfg, corners, waves, cam[4:7], 123.5, texels, True, fmt=fmt, wrap=wrap)
+    print(res, flush=True)
+    assert res["pos_max"] <= POS_TOL and res["nrm_max"] <= NRM_TOL
+    assert res["n_excluded"] <= 257 * 257 // 50
+
+
 def test_long_time_far_field(torch_cuda, wtctx):
     """t = 3600 s and corners up to ~1.5 km: the FP64 phase reduction keeps FP32 parity."""
     say("t=3600 far field")
@@ -179,6 +201,28 @@ def test_long_time_far_field(torch_cuda, wtctx):
     res = check_parity(torch_cuda, wtctx, cfg, corners, waves, cam[4:7], 3599.75, None, False)
     print(res, flush=True)
     assert res["pos_max"] <= POS_TOL and res["nrm_max"] <= NRM_TOL
+
+
+@pytest.mark.parametrize("direction", ["across_rows", "along_rows"])
+@pytest.mark.parametrize("shape", [(2, 16, 64), (1, 16, 8)])
+def test_recurrence_paths(torch_cuda, wtctx, direction, shape):
+    """The longest recurrences (K = 16) with every wave across the grid rows (phase step
+    along a row near 0: the rotation form) or along them (mostly the three-term form),
+    at t = 3600 s."""
+    say("recurrence paths %s shape %s" % (direction, shape))
+    cam = scenes.camera_tuple(height=6.0, pitch_deg=-20.0, yaw_deg=-90.0)
+    corners = wt.projector_corners(cam, 2.0)
+    waves = scenes.gen_waves(41, 32).copy()
+    waves[:, 4:6] = (0.0, 1.0) if direction == "across_rows" else (1.0, 0.0)
+    cfg = dict(N=1024, sample_scale=0.0, displacement_scale=0.0, scroll=(0.0, 0.0))
+    wtctx.set_launch_config(*shape, 64)
+    try:
+        res = check_parity(torch_cuda, wtctx, cfg, corners, waves, cam[4:7], 3600.0, None, True)
+    finally:
+        wtctx.set_launch_config(0, 0, 0, 0)
+    print(res, flush=True)
+    assert res["pos_max"] <= POS_TOL and res["nrm_max"] <= NRM_TOL
+    assert res["tan_max"] <= NRM_TOL and res["bit_max"] <= NRM_TOL
 
 
 def test_graph_replay_matches_eager(torch_cuda, wtctx):
