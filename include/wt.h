@@ -228,6 +228,10 @@ wt_status wt_batch_create(wt_ctx* ctx, const wt_patch* patches, uint32_t count, 
 void wt_batch_destroy(wt_batch* batch);
 /* Total vertices of the batch (the size the output arrays need, in float3). */
 wt_status wt_batch_vertex_count(const wt_batch* batch, uint64_t* vertices);
+/* The segment shape the batch runs patch `patch` with (chosen per patch at creation
+ * to minimise padded vertices; no reference counterpart). wt_set_launch_config with the
+ * same shape makes wt_water_displace on that patch alone produce identical bits. */
+wt_status wt_batch_patch_launch_config(const wt_batch* batch, uint32_t patch, wt_launch_config* cfg);
 /* One persistent launch over every patch: each patch's vertex program at
  * time + time_offset. d_waves holds total_waves waves that the patches index into.
  * Replayable from a captured graph (the work counter resets itself). */

@@ -31,25 +31,25 @@ def run_batch(torch, ctx, specs, texels, time_offset=0.0):
     return batch, patches, waves, w, hm, pos, nrm
 
 
-def batch_shape(specs):
-    return (2, 8, 64) if max(s["N"] for s in specs) > 256 else (1, 4, 32)
-
-
 @pytest.mark.parametrize("count,seed", [(24, 5), (7, 6)])
 def test_batched_equals_plain_per_patch(torch_cuda, wtctx, count, seed):
     """Every patch of a batch equals the plain kernel run on that patch alone with the
-    same launch shape, bit for bit; so the batch inherits the plain kernel's parity."""
+    launch shape the batch chose for it, bit for bit; so the batch inherits the plain
+    kernel's parity."""
     torch = torch_cuda
     specs = scenes.batched_patches(seed=seed, count=count)
-    if seed == 6:  # a batch of small grids only (the narrow shape)
+    if seed == 6:  # a batch of small grids only (the narrow shapes)
         for s in specs:
             s["N"] = max(2, s["N"] // 8)
     say("batched vs plain: %d patches, sides %s" % (count, [s["N"] for s in specs]))
     texels = scenes.value_noise(3, 128)
     batch, patches, waves, w, hm, pos, nrm = run_batch(torch, wtctx, specs, texels, time_offset=0.5)
-    wtctx.set_launch_config(*batch_shape(specs), 64)
+    shapes = set()
     try:
         for i, (s, p) in enumerate(zip(specs, patches)):
+            shape = batch.patch_launch_config(i)
+            shapes.add(shape)
+            wtctx.set_launch_config(*shape)
             N = s["N"]
             params = wt.water_params(np.ctypeslib.as_array(p.corners).copy(), s["camera"][4:7], N, s["time"] + 0.5, 0.02, 0.05, 0.0,
                                      (0.3, 0.0))
@@ -63,6 +63,8 @@ def test_batched_equals_plain_per_patch(torch_cuda, wtctx, count, seed):
             assert torch.equal(Nn, nrm[o:o + N * N * 3]), i
     finally:
         wtctx.set_launch_config()
+    print("shapes used:", sorted(shapes), flush=True)
+    assert len(shapes) >= 2  # both batches mix segment shapes
     batch.close()
 
 

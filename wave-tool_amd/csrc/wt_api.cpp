@@ -452,7 +452,6 @@ struct wt_batch {
   int n_units = 0;
   uint64_t vertices = 0;
   uint32_t max_wave_end = 0;
-  wt::LaunchShape shape{};
 };
 
 extern "C" {
@@ -499,22 +498,28 @@ wt_status wt_batch_create(wt_ctx* ctx, const wt_patch* patches, uint32_t count, 
     d.W = (int)p.wave_count;
     d.wave_offset = (int)p.wave_offset;
     d.vertex_offset = (long long)p.vertex_offset;
+    d.shape = wt::batch_shape_for(d.N);
     b->patches.push_back(d);
     b->vertices = std::max<uint64_t>(b->vertices, p.vertex_offset + (uint64_t)p.gridLength * p.gridLength);
     b->max_wave_end = std::max<uint32_t>(b->max_wave_end, p.wave_offset + p.wave_count);
     max_n = std::max(max_n, p.gridLength);
   }
-  // one shape for the whole batch: the wide shape when any grid is wider than 256
-  b->shape = max_n > 256 ? wt::LaunchShape{2, 8, 64, 64} : wt::LaunchShape{1, 4, 32, 64};
+  // units: one workgroup's segments of one patch, in that patch's segment shape
   std::vector<wt::BatchUnit> units;
   for (uint32_t i = 0; i < count; ++i) {
-    const long long groups = wt::forward_segment_groups(b->patches[i].N, b->shape);
+    const wt::PatchDesc& d = b->patches[i];
+    const wt::LaunchShape& s = wt::kBatchShape[d.shape];
+    const long long groups = wt::forward_segment_groups(d.N, s);
     for (long long gidx = 0; gidx < groups; ++gidx) units.push_back(wt::BatchUnit{(int)i, (int)gidx});
   }
-  // most expensive units first (a unit costs about its patch's wave count)
-  std::stable_sort(units.begin(), units.end(), [&](const wt::BatchUnit& x, const wt::BatchUnit& y) {
-    return b->patches[x.patch].W > b->patches[y.patch].W;
-  });
+  // most expensive units first: a unit costs about wave count x vertices per segment
+  auto cost = [&](const wt::BatchUnit& x) {
+    const wt::PatchDesc& d = b->patches[x.patch];
+    const wt::LaunchShape& s = wt::kBatchShape[d.shape];
+    return (double)d.W * s.lanes_per_row * s.steps * (64 / s.lanes_per_row);
+  };
+  std::stable_sort(units.begin(), units.end(),
+                   [&](const wt::BatchUnit& x, const wt::BatchUnit& y) { return cost(x) > cost(y); });
   b->n_units = (int)units.size();
   hipError_t e = hipMalloc(&b->d_patches, sizeof(wt::PatchDesc) * count);
   if (e == hipSuccess) e = hipMalloc(&b->d_units, sizeof(wt::BatchUnit) * units.size());
@@ -544,6 +549,16 @@ void wt_batch_destroy(wt_batch* b) {
 wt_status wt_batch_vertex_count(const wt_batch* b, uint64_t* vertices) {
   if (!b || !vertices) return WT_ERR_INVALID_ARG;
   *vertices = b->vertices;
+  return WT_OK;
+}
+
+wt_status wt_batch_patch_launch_config(const wt_batch* b, uint32_t patch, wt_launch_config* cfg) {
+  if (!b || !cfg || patch >= b->patches.size()) return WT_ERR_INVALID_ARG;
+  const wt::LaunchShape& s = wt::kBatchShape[b->patches[patch].shape];
+  cfg->wave_split = (uint32_t)s.split;
+  cfg->steps_per_lane = (uint32_t)s.steps;
+  cfg->lanes_per_row = (uint32_t)s.lanes_per_row;
+  cfg->wave_chunk = (uint32_t)s.wave_chunk;
   return WT_OK;
 }
 
@@ -587,10 +602,9 @@ wt_status wt_water_displace_batched(wt_ctx* ctx, wt_batch* b, const wt_gerstner_
   a.bit = out->bitangent;
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || cus <= 0) cus = 256;
-  const long long resident =
-      (long long)wt::batched_blocks_per_cu(b->shape, wt::tex_mode(a.texels, a.tex_fmt, a.tex_wrap)) * cus;
+  const long long resident = (long long)wt::batched_blocks_per_cu(wt::tex_mode(a.texels, a.tex_fmt, a.tex_wrap)) * cus;
   const int grid = (int)std::min<long long>(resident, b->n_units);
-  hipError_t e = wt::launch_forward_batched(a, b->shape, grid, st);
+  hipError_t e = wt::launch_forward_batched(a, grid, st);
   if (e != hipSuccess) return hip_fail(ctx, e, "water_forward_batched_kernel launch");
   return WT_OK;
 }

@@ -28,51 +28,57 @@ long long forward_segment_groups(int N, const LaunchShape& s) {
   return (segs + per_wg - 1) / per_wg;
 }
 
-// The batched kernel is built for two shapes: (K 8, LC 64, S 2) for grids wider than
-// 256 and (K 4, LC 32, S 1) otherwise; the batch picks one for all its patches.
-template <int K, int LC, int S>
-static hipError_t launch_batched_t(const BatchArgs& a, int grid, hipStream_t st) {
+// Relative cost of a vertex-wave in the K = 4 shapes (one seed per 4 steps instead of 8,
+// and more step-table rows); 1.3 was fastest on the 512-patch workload among 1.1-2.0.
+constexpr double kBatchK4Premium = 1.3;
+
+int batch_shape_for(int N) {
+  // padded vertices per shape, K = 4 shapes weighted by their larger seed share
+  double best = 0.0;
+  int pick = 0;
+  for (int i = 0; i < kBatchShapes; ++i) {
+    const LaunchShape& s = kBatchShape[i];
+    const long long cols = s.lanes_per_row * s.steps, rows = 64 / s.lanes_per_row;
+    const double padded = (double)((N + cols - 1) / cols * cols) * (double)((N + rows - 1) / rows * rows);
+    const double cost = padded * (s.steps == 8 ? 1.0 : kBatchK4Premium);
+    if (i == 0 || cost < best) {
+      best = cost;
+      pick = i;
+    }
+  }
+  return pick;
+}
+
+hipError_t launch_forward_batched(const BatchArgs& a, int grid, hipStream_t st) {
   const dim3 g(grid), b(256);
   switch (tex_mode(a.texels, a.tex_fmt, a.tex_wrap)) {
-    case kTexR32Repeat: hipLaunchKernelGGL((water_forward_batched_kernel<K, LC, S, kTexR32Repeat>), g, b, 0, st, a); break;
-    case kTexR32Mirror: hipLaunchKernelGGL((water_forward_batched_kernel<K, LC, S, kTexR32Mirror>), g, b, 0, st, a); break;
-    case kTexR8Repeat: hipLaunchKernelGGL((water_forward_batched_kernel<K, LC, S, kTexR8Repeat>), g, b, 0, st, a); break;
-    case kTexR8Mirror: hipLaunchKernelGGL((water_forward_batched_kernel<K, LC, S, kTexR8Mirror>), g, b, 0, st, a); break;
-    default: hipLaunchKernelGGL((water_forward_batched_kernel<K, LC, S, kTexNone>), g, b, 0, st, a); break;
+    case kTexR32Repeat: hipLaunchKernelGGL((water_forward_batched_kernel<kTexR32Repeat>), g, b, 0, st, a); break;
+    case kTexR32Mirror: hipLaunchKernelGGL((water_forward_batched_kernel<kTexR32Mirror>), g, b, 0, st, a); break;
+    case kTexR8Repeat: hipLaunchKernelGGL((water_forward_batched_kernel<kTexR8Repeat>), g, b, 0, st, a); break;
+    case kTexR8Mirror: hipLaunchKernelGGL((water_forward_batched_kernel<kTexR8Mirror>), g, b, 0, st, a); break;
+    default: hipLaunchKernelGGL((water_forward_batched_kernel<kTexNone>), g, b, 0, st, a); break;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_forward_batched(const BatchArgs& a, const LaunchShape& s, int grid, hipStream_t st) {
-  if (s.steps == 8 && s.lanes_per_row == 64 && s.split == 2) return launch_batched_t<8, 64, 2>(a, grid, st);
-  return launch_batched_t<4, 32, 1>(a, grid, st);
-}
-
-template <int K, int LC, int S, int kTex>
+template <int kTex>
 static int occupancy_kernel() {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, water_forward_batched_kernel<K, LC, S, kTex>, 256, 0) !=
-      hipSuccess) {
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, water_forward_batched_kernel<kTex>, 256, 0) != hipSuccess) {
     (void)hipGetLastError();
     return 1;
   }
   return n > 0 ? n : 1;
 }
 
-template <int K, int LC, int S>
-static int occupancy_t(int tex) {
+int batched_blocks_per_cu(int tex) {
   switch (tex) {
-    case kTexR32Repeat: return occupancy_kernel<K, LC, S, kTexR32Repeat>();
-    case kTexR32Mirror: return occupancy_kernel<K, LC, S, kTexR32Mirror>();
-    case kTexR8Repeat: return occupancy_kernel<K, LC, S, kTexR8Repeat>();
-    case kTexR8Mirror: return occupancy_kernel<K, LC, S, kTexR8Mirror>();
-    default: return occupancy_kernel<K, LC, S, kTexNone>();
+    case kTexR32Repeat: return occupancy_kernel<kTexR32Repeat>();
+    case kTexR32Mirror: return occupancy_kernel<kTexR32Mirror>();
+    case kTexR8Repeat: return occupancy_kernel<kTexR8Repeat>();
+    case kTexR8Mirror: return occupancy_kernel<kTexR8Mirror>();
+    default: return occupancy_kernel<kTexNone>();
   }
-}
-
-int batched_blocks_per_cu(const LaunchShape& s, int tex) {
-  if (s.steps == 8 && s.lanes_per_row == 64 && s.split == 2) return occupancy_t<8, 64, 2>(tex);
-  return occupancy_t<4, 32, 1>(tex);
 }
 
 }  // namespace wt

@@ -35,6 +35,7 @@
 namespace wt {
 
 constexpr int kChunk = 64;      // waves staged per LDS pass
+constexpr size_t cmax(size_t a, size_t b) { return a > b ? a : b; }
 constexpr int kWavesPerWG = 4;  // 4 wavefronts of 64 = 256 threads
 
 // FP64 row geometry: x(col) = xa + col * xs, z(col) = za + col * zs.
@@ -383,11 +384,15 @@ __device__ __forceinline__ FwdParams patch_params(const BatchArgs& a, const Patc
 
 // Persistent kernel over a batch of patches: each workgroup takes the next unit (one
 // workgroup's segments of one patch; units are sorted by cost, largest first) from a
-// device counter until the list is exhausted. The last workgroup to finish resets the
-// counters, so the launch can be replayed from a captured graph without a memset node.
-template <int K, int LC, int S, int kTex>
+// device counter until the list is exhausted, and runs it with its patch's segment shape
+// (kBatchShape). The last workgroup to finish resets the counters, so the launch can be
+// replayed from a captured graph without a memset node.
+constexpr size_t kBatchLds = cmax(cmax(sizeof(SegShared<8, 64, 2>), sizeof(SegShared<4, 32, 2>)),
+                                  sizeof(SegShared<4, 16, 2>));
+
+template <int kTex>
 __global__ __launch_bounds__(256) void water_forward_batched_kernel(BatchArgs a) {
-  __shared__ SegShared<K, LC, S> sh;
+  __shared__ __attribute__((aligned(16))) unsigned char raw[kBatchLds];
   __shared__ int s_unit;
   for (;;) {
     if (threadIdx.x == 0) s_unit = (int)atomicAdd(&a.counters[0], 1u);
@@ -396,8 +401,19 @@ __global__ __launch_bounds__(256) void water_forward_batched_kernel(BatchArgs a)
     __syncthreads();  // every thread has read s_unit before it is overwritten
     if (u >= a.n_units) break;
     const BatchUnit unit = a.units[u];
-    const FwdParams p = patch_params(a, a.patches[unit.patch]);
-    forward_segments<K, LC, S, kTex>(p, (long long)unit.group, sh);
+    const PatchDesc& pd = a.patches[unit.patch];
+    const FwdParams p = patch_params(a, pd);
+    switch (pd.shape) {
+      case 0:
+        forward_segments<8, 64, 2, kTex>(p, (long long)unit.group, *reinterpret_cast<SegShared<8, 64, 2>*>(raw));
+        break;
+      case 1:
+        forward_segments<4, 32, 2, kTex>(p, (long long)unit.group, *reinterpret_cast<SegShared<4, 32, 2>*>(raw));
+        break;
+      default:
+        forward_segments<4, 16, 2, kTex>(p, (long long)unit.group, *reinterpret_cast<SegShared<4, 16, 2>*>(raw));
+        break;
+    }
   }
   if (threadIdx.x == 0) {
     __threadfence();

@@ -37,7 +37,8 @@ struct PatchDesc {
   double time;
   float cam[3];
   float s_h, s_d, bounce, scroll_x, scroll_z;
-  int N, W, wave_offset, reserved;
+  int N, W, wave_offset;
+  int shape;                // segment shape of this patch in the batched kernel (kBatchShapes)
   long long vertex_offset;  // first vertex of the patch in the output arrays
 };
 
@@ -64,9 +65,18 @@ struct BatchArgs {
 
 // Segments a launch shape covers per workgroup for a grid of side N.
 long long forward_segment_groups(int N, const LaunchShape& s);
+
+// Segment shapes the batched kernel chooses from, per patch (all with S = 2): 512 x 1
+// (K = 8) for wide grids, 128 x 2 and 64 x 4 (K = 4) for narrower ones.
+constexpr int kBatchShapes = 3;
+constexpr LaunchShape kBatchShape[kBatchShapes] = {{2, 8, 64, 64}, {2, 4, 32, 64}, {2, 4, 16, 64}};
+// The shape with the fewest padded vertex-waves for a grid of side N (a small premium
+// on the K = 4 shapes for their extra seeding).
+int batch_shape_for(int N);
+
 // Launches the persistent batched kernel with `grid` workgroups.
-hipError_t launch_forward_batched(const BatchArgs& a, const LaunchShape& s, int grid, hipStream_t st);
-// Workgroups of the batched kernel resident per CU for this shape.
-int batched_blocks_per_cu(const LaunchShape& s, int tex_mode);
+hipError_t launch_forward_batched(const BatchArgs& a, int grid, hipStream_t st);
+// Workgroups of the batched kernel resident per CU.
+int batched_blocks_per_cu(int tex_mode);
 
 }  // namespace wt

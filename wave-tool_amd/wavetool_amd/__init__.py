@@ -211,6 +211,12 @@ class Batch:
                                              float(time_offset), _stream(stream))
         self.ctx._check(st)
 
+    def patch_launch_config(self, i):
+        """(wave_split, steps, lanes_per_row, wave_chunk) the batch runs patch i with."""
+        cfg = LaunchConfig()
+        check(lib().wt_batch_patch_launch_config(self.handle, int(i), ctypes.byref(cfg)))
+        return (cfg.wave_split, cfg.steps_per_lane, cfg.lanes_per_row, cfg.wave_chunk)
+
     def close(self):
         if self.handle:
             lib().wt_batch_destroy(self.handle)
