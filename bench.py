@@ -103,7 +103,11 @@ def bench_plain(torch, wt, scenes, args):
     pos = torch.empty(V * 3, dtype=torch.float32, device=dev)
     nrm = torch.empty_like(pos)
     ctx = wt.Context()
-    if not args.no_autotune:
+    if args.launch_shape:
+        split, steps, lanes = (int(v) for v in args.launch_shape.split(","))
+        ctx.set_launch_config(split, steps, lanes, 64)
+        log("launch shape forced to split %d, steps %d, lanes per row %d" % (split, steps, lanes))
+    elif not args.no_autotune:
         shape, ms = ctx.autotune(N, W, True)
         log("autotuned launch shape %s (%.3f ms per frame)" % (shape, ms))
     shape = ctx.get_launch_config(N, W)
@@ -294,6 +298,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--launch-shape", default="", help="SPLIT,STEPS,LANES: fixed forward launch shape, no autotune")
     ap.add_argument("--no-extras", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
