@@ -194,6 +194,49 @@ def bench_batched_forward(torch, wt, scenes, frames=120):
                 speedup_vs_plain=round(batched / (vw_frame / pwall), 2))
 
 
+# --------------------------------------------------------------------- varyings
+def bench_varyings(torch, wt, scenes, launches=100):
+    """The varyings pass (clip position, height-0 NDC, view vector, yaw-space normal,
+    heightmap sample) over the config-1 grid: an HBM-bound pass, 24 B read and 52 B
+    written per vertex, timed as a captured graph of `launches` launches."""
+    dev = torch.device("cuda:0")
+    cfg = scenes.plain_config(1)
+    N, W = cfg["N"], cfg["W"]
+    V = N * N
+    cam = cfg["camera"]
+    ctx = wt.Context()
+    corners = wt.projector_corners(cam, cfg["amp"])
+    waves = torch.from_numpy(scenes.gen_waves(cfg["seed"], W)).to(dev)
+    hm = torch.from_numpy(scenes.value_noise(cfg["heightmap"]["seed"], cfg["heightmap"]["size"])).to(dev)
+    params = wt.water_params(corners, cam[4:7], N, cfg["t0"], cfg["sample_scale"], cfg["displacement_scale"], 0.0,
+                             cfg["scroll"])
+    pos = torch.empty(V * 3, device=dev)
+    nrm = torch.empty_like(pos)
+    ctx.displace(params, waves, pos, nrm, heightmap=hm)
+    view, proj, _ = wt.camera_matrices(cam)
+    view_yaw, _, _ = wt.camera_matrices(cam[:8] + (0.0,))
+    outs = dict(clip=torch.empty(V * 4, device=dev), ndc0=torch.empty(V * 2, device=dev),
+                view=torch.empty(V * 3, device=dev), nview=torch.empty(V * 3, device=dev),
+                sample=torch.empty(V, device=dev))
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    ctx.graph_begin(stream)
+    for _ in range(launches):
+        ctx.varyings(params, proj @ view, view_yaw, pos, nrm, V, heightmap=hm, stream=stream, **outs)
+    graph = ctx.graph_end(stream)
+    graph.launch(stream)
+    _, ev = timed(torch, stream, lambda: graph.launch(stream))
+    graph.close()
+    ctx.close()
+    t = ev / launches
+    traffic = 76 * V
+    return dict(workload="varyings of the config-1 grid (1024 x 1024): clip position, height-0 NDC xy, view vector, "
+                         "yaw-only view-space normal, heightmap sample",
+                value=V / t, unit="vertices/s", launch_us=round(t * 1e6, 3),
+                roofline=dict(bound="hbm", achieved=round(traffic / t / 1e9, 1), peak=8000.0, unit="GB/s",
+                              frac=round(traffic / t / 8e12, 4), bytes_per_vertex=76))
+
+
 # --------------------------------------------------------------------- configs 3 and 4
 def make_fits(torch, wt, scenes, ctx, specs, sigma=0.01):
     """Device targets (synthesized from the hidden waves) and perturbed initial params."""
@@ -320,6 +363,9 @@ def main():
         extras = {}
         extras["batched_forward"] = bench_batched_forward(torch, wt, scenes)
         log("config2 %.4g vertex-waves/s" % extras["batched_forward"]["value"])
+        extras["varyings"] = bench_varyings(torch, wt, scenes)
+        log("varyings %.4g vertices/s, %.1f GB/s" % (extras["varyings"]["value"],
+                                                      extras["varyings"]["roofline"]["achieved"]))
         extras["fit_single"] = bench_fit_single(torch, wt, scenes, cpu=not args.no_cpu_baseline)
         log("config3 %.4g fits/s" % extras["fit_single"]["value"])
         extras["fit_batched"] = bench_fit_batched(torch, wt, scenes)

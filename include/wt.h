@@ -185,6 +185,27 @@ wt_status wt_water_displace(wt_ctx* ctx, const wt_water_params* params, const wt
                             uint32_t wave_count, const wt_heightmap* heightmap, const wt_vertex_buffers* out,
                             void* stream);
 
+/* The water program's remaining outputs (water-grid.vert:66-70, 141-158, 193): the
+ * varyings its fragment stage reads, formed from the positions and normals
+ * wt_water_displace wrote. Each array is optional (NULL skips it). */
+typedef struct wt_varying_buffers {
+  float* clip_position;        /* float4: viewProjection * (P, 1), the vertex's gl_Position */
+  float* ndc_xy_height0;       /* float2: xy / w of viewProjection * (P.x, 0, P.z, 1) */
+  float* view_vector;          /* float3: cameraPosition - P (viewVecRaw, not normalised) */
+  float* normal_view_only_yaw; /* float3: normalize(viewMatOnlyYaw * (N, 0)) */
+  float* heightmap_sample;     /* float: the heightmap value sampled at P (heightmap_colour.r) */
+} wt_varying_buffers;
+
+/* One HBM-bound pass over vertex_count vertices (positions / normals as float3 arrays).
+ * view_projection = P V of the camera and view_only_yaw = V of the same camera with
+ * pitch 0 (src/render-engine.cpp:278-284), both 4x4 column-major floats as the
+ * reference uploads them (wt_camera_matrices gives V and P in double). params supplies
+ * cameraPosition, the heightmap scales, the scroll velocity and the time; heightmap may
+ * be NULL only when heightmap_sample is NULL. */
+wt_status wt_vertex_varyings(wt_ctx* ctx, const wt_water_params* params, const wt_heightmap* heightmap,
+                             const float view_projection[16], const float view_only_yaw[16], const float* positions,
+                             const float* normals, uint64_t vertex_count, const wt_varying_buffers* out, void* stream);
+
 /* Overrides (or with all-zero fields, restores) the launch shape of the plain
  * forward kernel. Returns WT_ERR_INVALID_ARG for unsupported values. */
 wt_status wt_set_launch_config(wt_ctx* ctx, const wt_launch_config* cfg);

@@ -1,0 +1,111 @@
+"""GPU tests of the varyings pass (wt_vertex_varyings): the water program's clip-space
+position, height-0 NDC xy, view vector, yaw-only view-space normal and heightmap sample
+(assets/shaders/water-grid.vert:141-158, 193), checked against FP64 evaluations of the
+same formulas on the same positions and normals, and the sample against the oracle's
+bilinear filter."""
+import numpy as np
+import pytest
+
+import oracle as orc
+import wavetool_amd as wt
+from wavetool_amd import scenes
+
+pytestmark = pytest.mark.gpu
+EPS32 = np.finfo(np.float32).eps
+
+
+def say(msg):
+    print("[case] " + msg, flush=True)
+
+
+@pytest.mark.parametrize("fmt,wrap", [(wt.WT_TEXEL_R32F, wt.WT_WRAP_REPEAT),
+                                      (wt.WT_TEXEL_R8_UNORM, wt.WT_WRAP_MIRRORED_REPEAT)])
+def test_varyings_match_fp64(torch_cuda, wtctx, fmt, wrap):
+    torch = torch_cuda
+    say("varyings fmt=%d wrap=%d" % (fmt, wrap))
+    dev = torch.device("cuda")
+    cam = scenes.camera_tuple(height=8.0, pitch_deg=-18.0, yaw_deg=-60.0)
+    corners = wt.projector_corners(cam, 2.0)
+    N, t = 257, 42.5
+    V = N * N
+    waves = torch.from_numpy(scenes.gen_waves(7, 24)).to(dev)
+    if fmt == wt.WT_TEXEL_R8_UNORM:
+        tex_np = np.random.default_rng(1).integers(0, 256, size=(48, 64), dtype=np.uint8)
+    else:
+        tex_np = scenes.value_noise(4, 64)
+    tex = torch.from_numpy(tex_np).to(dev)
+    s_h, scroll = 0.03, (0.3, -0.2)
+    params = wt.water_params(corners, cam[4:7], N, t, s_h, 0.1, 0.0, scroll)
+    pos = torch.empty(V * 3, device=dev)
+    nrm = torch.empty_like(pos)
+    wtctx.displace(params, waves, pos, nrm, heightmap=tex, hm_format=fmt, hm_wrap=wrap)
+    view, proj, _ = wt.camera_matrices(cam)
+    view_yaw, _, _ = wt.camera_matrices(cam[:8] + (0.0,))
+    vp = proj @ view
+    # the kernel takes FP32 matrices, as the reference uploads them
+    vp32, vy32 = vp.astype(np.float32).astype(np.float64), view_yaw.astype(np.float32).astype(np.float64)
+    clip = torch.empty(V * 4, device=dev)
+    ndc0 = torch.empty(V * 2, device=dev)
+    vvec = torch.empty(V * 3, device=dev)
+    nview = torch.empty(V * 3, device=dev)
+    sample = torch.empty(V, device=dev)
+    wtctx.varyings(params, vp, view_yaw, pos, nrm, V, clip=clip, ndc0=ndc0, view=vvec, nview=nview, sample=sample,
+                   heightmap=tex, hm_format=fmt, hm_wrap=wrap)
+    torch.cuda.synchronize()
+    P = pos.cpu().numpy().reshape(V, 3).astype(np.float64)
+    Nn = nrm.cpu().numpy().reshape(V, 3).astype(np.float64)
+    ph = np.concatenate([P, np.ones((V, 1))], axis=1)
+    # clip = VP (P, 1): FP32 dot products of 4 terms
+    ref = ph @ vp32.T
+    bound = 8 * EPS32 * (np.abs(ph) @ np.abs(vp32).T) + 1e-30
+    got = clip.cpu().numpy().reshape(V, 4)
+    assert (np.abs(got - ref) <= bound).all()
+    # NDC xy of the height-0 point
+    p0 = ph.copy()
+    p0[:, 1] = 0.0
+    c0 = p0 @ vp32.T
+    ok = np.abs(c0[:, 3]) > 1e-2
+    ref0 = c0[:, :2] / c0[:, 3:4]
+    got0 = ndc0.cpu().numpy().reshape(V, 2)
+    b0 = (8 * EPS32 * (np.abs(p0) @ np.abs(vp32).T))
+    tol0 = (b0[:, :2] + np.abs(ref0) * b0[:, 3:4]) / np.abs(c0[:, 3:4]) + 4 * EPS32 * np.abs(ref0)
+    assert ok.mean() > 0.99 and (np.abs(got0 - ref0)[ok] <= tol0[ok]).all()
+    # view vector: one FP32 subtraction per component
+    refv = np.asarray(cam[4:7])[None, :] - P
+    gotv = vvec.cpu().numpy().reshape(V, 3)
+    assert (np.abs(gotv - refv) <= EPS32 * np.maximum(np.abs(refv), 1e-30) + 1e-30).all()
+    # yaw-only view-space normal: a unit vector within 1e-6 rad
+    refn = Nn @ vy32[:3, :3].T
+    refn /= np.linalg.norm(refn, axis=1, keepdims=True)
+    gotn = nview.cpu().numpy().reshape(V, 3).astype(np.float64)
+    ang = np.arctan2(np.linalg.norm(np.cross(gotn, refn), axis=1), (gotn * refn).sum(1))
+    assert ang.max() <= 1e-6 and np.abs(np.linalg.norm(gotn, axis=1) - 1).max() <= 1e-6
+    # heightmap sample at the displaced point, against the oracle's bilinear filter
+    idx = np.random.default_rng(2).choice(V, 1500, replace=False)
+    sh, sx, sz = (float(np.float32(v)) for v in (s_h, scroll[0], scroll[1]))  # the ABI carries them in FP32
+    s = sh * P[idx, 0] - sh * sx * t
+    tt = -sh * P[idx, 2] + sh * sz * t
+    refs = np.array([orc.sample_bilinear(tex_np, fmt, wrap, a, b)[0] for a, b in zip(s, tt)])
+    gots = sample.cpu().numpy()[idx]
+    print("sample max err", np.abs(gots - refs).max(), flush=True)
+    assert np.abs(gots - refs).max() <= 2e-5
+
+
+def test_varyings_reject_bad_arguments(torch_cuda, wtctx):
+    torch = torch_cuda
+    dev = torch.device("cuda")
+    cam = scenes.camera_tuple()
+    params = wt.water_params(wt.projector_corners(cam, 2.0), cam[4:7], 4, 0.0)
+    pos = torch.zeros(48, device=dev)
+    eye = np.eye(4)
+    with pytest.raises(wt.WtError):  # heightmap_sample without a heightmap
+        wtctx.varyings(params, eye, eye, pos, pos, 16, sample=torch.empty(16, device=dev))
+    with pytest.raises(wt.WtError):  # misaligned float4 output
+        clip = torch.empty(16 * 4 + 1, device=dev)[1:]
+        wtctx.varyings(params, eye, eye, pos, pos, 16, clip=clip)
+    with pytest.raises(wt.WtError):  # non-finite matrix
+        bad = eye.copy()
+        bad[0, 0] = np.nan
+        wtctx.varyings(params, bad, eye, pos, pos, 16, view=torch.empty(48, device=dev))
+    # nothing requested: a no-op
+    wtctx.varyings(params, eye, eye, pos, pos, 16)

@@ -7,6 +7,8 @@ import subprocess
 import numpy as np
 import pytest
 
+import wavetool_amd as wt
+
 from helpers import NRM_TOL, POS_TOL, compare, near_texel_edge
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -51,12 +53,22 @@ def test_host_stage_default_scene(tmp_path):
     N = 17
     nf = N * N * 3
     corners = np.frombuffer(raw[:128], dtype=np.float64).reshape(4, 4)
-    frames = np.frombuffer(raw[128:], dtype=np.float32).reshape(3, 4, N * N, 3)
+    per_frame = 4 * N * N * 3 + N * N * 4
+    body = np.frombuffer(raw[128:], dtype=np.float32).reshape(3, per_frame)
+    frames = body[:, :4 * N * N * 3].reshape(3, 4, N * N, 3)
+    clips = body[:, 4 * N * N * 3:].reshape(3, N * N, 4)
     with np.load(GOLDEN, allow_pickle=False) as z:
         g = {k: z[k] for k in z.files}
     # the stage forms the slab amplitude in FP32 like render-engine.cpp:803; the fixture in FP64
     assert np.abs(corners - g["corners"]).max() <= 1e-5 * np.abs(g["corners"]).max()
     assert frames.size == 3 * 4 * nf
+    # clip-space position of the stage's own positions: viewProjection of the reference camera
+    view, proj, _ = wt.camera_matrices((72.0, 1.0, 0.1, 100.0, 0.0, 4.0, 70.0, -90.0, 0.0))
+    vp32 = (proj @ view).astype(np.float32).astype(np.float64)
+    for f in range(3):
+        ph = np.concatenate([frames[f, 0].astype(np.float64), np.ones((N * N, 1))], axis=1)
+        bound = 8 * np.finfo(np.float32).eps * (np.abs(ph) @ np.abs(vp32).T) + 1e-30
+        assert (np.abs(clips[f] - ph @ vp32.T) <= bound).all()
     for f, t in enumerate((0.0, 10.0, 3600.0)):
         got = dict(zip(("pos", "nrm", "tan", "bit"), frames[f]))
         ref = {k: g["t%g_%s" % (t, k)] for k in ("pos", "nrm", "tan", "bit")}

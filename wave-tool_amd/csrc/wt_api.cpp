@@ -276,6 +276,67 @@ wt_status wt_water_displace(wt_ctx* ctx, const wt_water_params* p, const wt_gers
   return WT_OK;
 }
 
+wt_status wt_vertex_varyings(wt_ctx* ctx, const wt_water_params* p, const wt_heightmap* hm, const float vp[16],
+                             const float vyaw[16], const float* positions, const float* normals, uint64_t count,
+                             const wt_varying_buffers* out, void* stream) {
+  if (!ctx) return WT_ERR_INVALID_ARG;
+  if (!p || !out || !vp || !vyaw) return fail(ctx, WT_ERR_INVALID_ARG, "params, matrices and out must be non-NULL");
+  if (count > (1ull << 40)) return fail(ctx, WT_ERR_INVALID_ARG, "vertex_count too large");
+  for (int i = 0; i < 16; ++i)
+    if (!std::isfinite(vp[i]) || !std::isfinite(vyaw[i])) return fail(ctx, WT_ERR_INVALID_ARG, "matrices must be finite");
+  const float fs[6] = {p->cameraPosition[0], p->cameraPosition[1], p->cameraPosition[2], p->heightmapSampleScale,
+                       p->heightmapScrollVelocity[0], p->heightmapScrollVelocity[1]};
+  for (float v : fs)
+    if (!std::isfinite(v)) return fail(ctx, WT_ERR_INVALID_ARG, "scalar parameters must be finite");
+  if (!is_finite(p->waveAnimationTimeInSeconds)) return fail(ctx, WT_ERR_INVALID_ARG, "time must be finite");
+  const bool needs_pos = out->clip_position || out->ndc_xy_height0 || out->view_vector || out->heightmap_sample;
+  if (needs_pos && !positions) return fail(ctx, WT_ERR_INVALID_ARG, "positions are required");
+  if (out->normal_view_only_yaw && !normals) return fail(ctx, WT_ERR_INVALID_ARG, "normals are required");
+  if (out->heightmap_sample && !hm) return fail(ctx, WT_ERR_INVALID_ARG, "heightmap_sample needs a heightmap");
+  // float4 / float2 stores need 16 / 8-byte alignment, the rest 4
+  if ((reinterpret_cast<uintptr_t>(out->clip_position) & 15) || (reinterpret_cast<uintptr_t>(out->ndc_xy_height0) & 7))
+    return fail(ctx, WT_ERR_INVALID_ARG, "clip_position must be 16-byte and ndc_xy_height0 8-byte aligned");
+  const void* bufs[7] = {positions, normals, out->clip_position, out->ndc_xy_height0, out->view_vector,
+                         out->normal_view_only_yaw, out->heightmap_sample};
+  for (const void* b : bufs)
+    if (b && (reinterpret_cast<uintptr_t>(b) & 3)) return fail(ctx, WT_ERR_INVALID_ARG, "buffers must be 4-byte aligned");
+  hipStream_t st = (hipStream_t)stream;
+  const bool capturing = stream_capturing(st);
+  wt_status hs = wt::validate_heightmap(ctx, out->heightmap_sample ? hm : nullptr, !capturing);
+  if (hs != WT_OK) return hs;
+  if (!capturing)
+    for (const void* b : bufs)
+      if (b && !is_device_pointer(b)) return fail(ctx, WT_ERR_INVALID_ARG, "buffers must be device memory");
+  wt::VaryingArgs a{};
+  for (int i = 0; i < 16; ++i) {
+    a.vp[i] = vp[i];
+    a.vyaw[i] = vyaw[i];
+  }
+  for (int k = 0; k < 3; ++k) a.cam[k] = p->cameraPosition[k];
+  a.s_h = p->heightmapSampleScale;
+  if (out->heightmap_sample) {
+    a.texels = hm->texels;
+    a.tex_w = (int)hm->width;
+    a.tex_h = (int)hm->height;
+    a.tex_fmt = (int)hm->format;
+    a.tex_wrap = (int)hm->wrap;
+    const double t = p->waveAnimationTimeInSeconds, sh = p->heightmapSampleScale;
+    a.hm_off_s = reduce_offset(-sh * (double)p->heightmapScrollVelocity[0] * t, a.tex_wrap);
+    a.hm_off_t = reduce_offset(sh * (double)p->heightmapScrollVelocity[1] * t, a.tex_wrap);
+  }
+  a.pos = positions;
+  a.nrm = normals;
+  a.count = (long long)count;
+  a.clip = out->clip_position;
+  a.ndc0 = out->ndc_xy_height0;
+  a.view = out->view_vector;
+  a.nview = out->normal_view_only_yaw;
+  a.sample = out->heightmap_sample;
+  hipError_t he = wt::launch_varyings(a, st);
+  if (he != hipSuccess) return hip_fail(ctx, he, "vertex_varyings_kernel launch");
+  return WT_OK;
+}
+
 wt_status wt_set_launch_config(wt_ctx* ctx, const wt_launch_config* cfg) {
   if (!ctx || !cfg) return WT_ERR_INVALID_ARG;
   if (!cfg->wave_split && !cfg->steps_per_lane && !cfg->lanes_per_row && !cfg->wave_chunk) {

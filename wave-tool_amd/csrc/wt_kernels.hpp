@@ -63,6 +63,26 @@ struct BatchArgs {
   float* bit;
 };
 
+// Varyings pass (wt_vertex_varyings): one thread per vertex.
+struct VaryingArgs {
+  float vp[16];    // viewProjection, column-major
+  float vyaw[16];  // viewMatOnlyYaw, column-major
+  float cam[3];
+  float s_h;
+  double hm_off_s, hm_off_t;  // scroll offsets reduced to one wrap period
+  const void* texels;
+  int tex_w, tex_h, tex_fmt, tex_wrap;
+  const float* pos;
+  const float* nrm;
+  long long count;
+  float* clip;
+  float* ndc0;
+  float* view;
+  float* nview;
+  float* sample;
+};
+hipError_t launch_varyings(const VaryingArgs& a, hipStream_t st);
+
 // Segments a launch shape covers per workgroup for a grid of side N.
 long long forward_segment_groups(int N, const LaunchShape& s);
 

@@ -163,4 +163,35 @@ wt_status WaterStage::update(const wt_camera& camera, const WaveSet& waves, doub
   return WT_OK;
 }
 
+wt_status WaterStage::varyings(const wt_camera& camera, double time, const WaterSettings& s,
+                               const wt_heightmap* heightmap, const wt_varying_buffers& out, void* stream) {
+  if (!ctx_) return fail(WT_ERR_INVALID_ARG, "WaterStage::varyings before create");
+  double view[16], proj[16], fwd[3], view_yaw[16], proj_yaw[16], fwd_yaw[3];
+  wt_camera yaw_only = camera;
+  yaw_only.pitch_deg = 0.0f;  // Camera cameraOnlyYaw; setRotation(yaw, 0) (render-engine.cpp:279-281)
+  if (wt_camera_matrices(&camera, view, proj, fwd) != WT_OK ||
+      wt_camera_matrices(&yaw_only, view_yaw, proj_yaw, fwd_yaw) != WT_OK)
+    return fail(WT_ERR_INVALID_ARG, "camera matrices");
+  float vp[16], vy[16];
+  for (int c = 0; c < 4; ++c)  // column-major: element (row r, column c) at [4c + r]
+    for (int r = 0; r < 4; ++r) {
+      double acc = 0.0;
+      for (int k = 0; k < 4; ++k) acc += proj[4 * k + r] * view[4 * c + k];
+      vp[4 * c + r] = (float)acc;
+      vy[4 * c + r] = (float)view_yaw[4 * c + r];
+    }
+  wt_water_params p;
+  std::memset(&p, 0, sizeof p);
+  for (int k = 0; k < 3; ++k) p.cameraPosition[k] = camera.position[k];
+  p.gridLength = grid_.gridLength();
+  p.heightmapSampleScale = s.heightmapSampleScale;
+  p.heightmapDisplacementScale = s.heightmapDisplacementScale;
+  p.heightmapScrollVelocity[0] = s.heightmapScrollVelocity[0];
+  p.heightmapScrollVelocity[1] = s.heightmapScrollVelocity[1];
+  p.waveAnimationTimeInSeconds = time;
+  const wt_status st = wt_vertex_varyings(ctx_, &p, heightmap, vp, vy, grid_.positions(), grid_.normals(),
+                                          grid_.vertexCount(), &out, stream);
+  return st == WT_OK ? WT_OK : fail(st, wt_last_error(ctx_));
+}
+
 }  // namespace wavetool

@@ -114,6 +114,12 @@ class WaterStage {
   // the previous frame's wave-block copy before overwriting it.
   wt_status update(const wt_camera& camera, const WaveSet& waves, double waveAnimationTimeInSeconds,
                    const WaterSettings& settings, const wt_heightmap* heightmap, void* stream);
+  // The water program's varyings (water-grid.vert:141-158, 193) for the grid's current
+  // positions and normals: viewProjection and the yaw-only view matrix of `camera`
+  // (src/render-engine.cpp:278-284), then wt_vertex_varyings into `out` (any field NULL
+  // skips it). Call after update() on the same stream.
+  wt_status varyings(const wt_camera& camera, double waveAnimationTimeInSeconds, const WaterSettings& settings,
+                     const wt_heightmap* heightmap, const wt_varying_buffers& out, void* stream);
   const WaterGrid& grid() const { return grid_; }
   const double* corners() const { return &corners_[0][0]; }
   const std::string& lastError() const { return err_; }

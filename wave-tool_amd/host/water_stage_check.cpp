@@ -6,7 +6,8 @@
 //                                    17 x 17 grid with tangents, a 16 x 16 8-bit MIRRORED_REPEAT
 //                                    heightmap, s_h 0.02, s_d 1.0, A_b 0.1 at bounce phase 0,
 //                                    frames at t = 0, 10, 3600 s. Writes the corners (4x4 f64)
-//                                    then per frame position, normal, tangent, bitangent (f32).
+//                                    then per frame position, normal, tangent, bitangent (f32)
+//                                    and the clip-space position (float4, f32).
 //   water_stage_check --indices <n>  prints the triangle index list of an n x n grid (no GPU)
 #include <hip/hip_runtime.h>
 
@@ -68,7 +69,11 @@ int main(int argc, char** argv) {
   if (!f) return fail("cannot open", argv[1]);
   bool corners_written = false;
   const size_t nf = (size_t)N * N * 3;
-  std::vector<float> host(nf);
+  std::vector<float> host(nf), host_clip((size_t)N * N * 4);
+  float* d_clip = nullptr;
+  if (hipMalloc((void**)&d_clip, host_clip.size() * sizeof(float)) != hipSuccess) return fail("clip buffer");
+  wt_varying_buffers vary{};
+  vary.clip_position = d_clip;
   for (double t : {0.0, 10.0, 3600.0}) {
     if (stage.update(cam, waves, t, settings, &hm, nullptr) != WT_OK) return fail("update", stage.lastError());
     if (hipDeviceSynchronize() != hipSuccess) return fail("hipDeviceSynchronize");
@@ -81,6 +86,10 @@ int main(int argc, char** argv) {
       if (hipMemcpy(host.data(), d, nf * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) return fail("readback");
       std::fwrite(host.data(), sizeof(float), nf, f);
     }
+    if (stage.varyings(cam, t, settings, &hm, vary, nullptr) != WT_OK) return fail("varyings", stage.lastError());
+    if (hipMemcpy(host_clip.data(), d_clip, host_clip.size() * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+      return fail("clip readback");
+    std::fwrite(host_clip.data(), sizeof(float), host_clip.size(), f);
   }
   std::fclose(f);
   // a camera looking up from high above misses the slab: the frame is skipped
@@ -88,6 +97,7 @@ int main(int argc, char** argv) {
   if (stage.update(up, waves, 0.0, settings, &hm, nullptr) != WT_ERR_NO_INTERSECTION)
     return fail("expected WT_ERR_NO_INTERSECTION");
   (void)hipFree(d_tex);
+  (void)hipFree(d_clip);
   std::printf("water_stage_check: ok\n");
   return 0;
 }

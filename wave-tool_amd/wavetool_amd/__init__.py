@@ -8,8 +8,8 @@ kernels in libwavetool.so.
 import ctypes
 
 from . import scenes
-from ._abi import (LaunchConfig, Camera, GerstnerWave, Heightmap, VertexBuffers, WaterParams, FitProblem, AdamConfig, Patch,
-                   WtError, check, lib,
+from ._abi import (LaunchConfig, Camera, GerstnerWave, Heightmap, VertexBuffers, VaryingBuffers, WaterParams,
+                   FitProblem, AdamConfig, Patch, WtError, check, lib,
                    WT_OK, WT_ERR_INVALID_ARG, WT_ERR_NO_INTERSECTION, WT_ERR_DEGENERATE, WT_WRAP_REPEAT,
                    WT_WRAP_MIRRORED_REPEAT, WT_TEXEL_R8_UNORM, WT_TEXEL_R32F, WT_FLOP_PER_VERTEX_WAVE,
                    WT_BYTES_PER_VERTEX_PN, WT_BYTES_PER_VERTEX_PNTB)
@@ -117,6 +117,26 @@ class Context:
             hm = Heightmap(heightmap.data_ptr(), w, h, hm_format, hm_wrap)
         st = lib().wt_water_displace(self.handle, ctypes.byref(params), _ptr(waves) if W else None, W,
                                      ctypes.byref(hm) if hm is not None else None, ctypes.byref(vb), _stream(stream))
+        self._check(st)
+
+    def varyings(self, params, view_projection, view_only_yaw, pos, nrm, vertex_count, clip=None, ndc0=None,
+                 view=None, nview=None, sample=None, heightmap=None, hm_format=WT_TEXEL_R32F, hm_wrap=WT_WRAP_REPEAT,
+                 stream=None):
+        """The water program's varyings from positions / normals (wt_vertex_varyings).
+        Matrices: 4x4 arrays in the usual row / column indexing (m[r][c]); they are
+        passed column-major as the reference uploads them."""
+        import numpy as np
+        f16 = ctypes.c_float * 16
+        vp = f16(*[float(v) for v in np.asarray(view_projection, dtype=np.float64).T.reshape(-1)])
+        vy = f16(*[float(v) for v in np.asarray(view_only_yaw, dtype=np.float64).T.reshape(-1)])
+        out = VaryingBuffers(_ptr(clip), _ptr(ndc0), _ptr(view), _ptr(nview), _ptr(sample))
+        hm = None
+        if heightmap is not None:
+            h, w = heightmap.shape
+            hm = Heightmap(heightmap.data_ptr(), w, h, hm_format, hm_wrap)
+        st = lib().wt_vertex_varyings(self.handle, ctypes.byref(params), ctypes.byref(hm) if hm is not None else None,
+                                      vp, vy, _ptr(pos), _ptr(nrm), int(vertex_count), ctypes.byref(out),
+                                      _stream(stream))
         self._check(st)
 
     def set_launch_config(self, wave_split=0, steps=0, lanes_per_row=0, wave_chunk=0):

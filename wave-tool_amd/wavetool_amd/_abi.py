@@ -86,6 +86,16 @@ class VertexBuffers(ctypes.Structure):
     ]
 
 
+class VaryingBuffers(ctypes.Structure):
+    _fields_ = [
+        ("clip_position", ctypes.c_void_p),
+        ("ndc_xy_height0", ctypes.c_void_p),
+        ("view_vector", ctypes.c_void_p),
+        ("normal_view_only_yaw", ctypes.c_void_p),
+        ("heightmap_sample", ctypes.c_void_p),
+    ]
+
+
 class LaunchConfig(ctypes.Structure):
     _fields_ = [
         ("wave_split", ctypes.c_uint32),
@@ -150,6 +160,9 @@ PROTOTYPES = [
     ("wt_water_displace", ctypes.c_int,
      [_vp, ctypes.POINTER(WaterParams), _vp, ctypes.c_uint32, ctypes.POINTER(Heightmap),
       ctypes.POINTER(VertexBuffers), _vp]),
+    ("wt_vertex_varyings", ctypes.c_int,
+     [_vp, ctypes.POINTER(WaterParams), ctypes.POINTER(Heightmap), ctypes.POINTER(ctypes.c_float),
+      ctypes.POINTER(ctypes.c_float), _vp, _vp, ctypes.c_uint64, ctypes.POINTER(VaryingBuffers), _vp]),
     ("wt_set_launch_config", ctypes.c_int, [_vp, ctypes.POINTER(LaunchConfig)]),
     ("wt_get_launch_config", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(LaunchConfig)]),
     ("wt_autotune_forward", ctypes.c_int,
