@@ -77,3 +77,63 @@ def test_host_stage_default_scene(tmp_path):
         print(t, res, flush=True)
         assert res["pos_max"] <= POS_TOL and res["nrm_max"] <= NRM_TOL
         assert res["tan_max"] <= NRM_TOL and res["bit_max"] <= NRM_TOL
+
+
+def stb_like_red(img):
+    """What the reference's loader hands the shader as .r: stb_image forces RGBA (grey ->
+    R = G = B, palette -> its RGB, 16-bit -> high byte) and flips rows so row 0 is the
+    image's bottom (src/render-engine.cpp:1310-1311)."""
+    if img.mode in ("I;16", "I;16B"):
+        a = (np.asarray(img, dtype=np.uint16) >> 8).astype(np.uint8)
+    else:
+        a = np.asarray(img.convert("RGBA"))[:, :, 0]
+    return a[::-1].copy()
+
+
+@pytest.mark.parametrize("mode", ["L", "LA", "RGB", "RGBA", "P", "I;16"])
+def test_heightmap_png_loader(tmp_path, mode):
+    from PIL import Image
+    if not os.path.exists(CHECK):
+        pytest.fail("water_stage_check is not built (run __graft_entry__.build())")
+    rng = np.random.default_rng(5)
+    h, w = 37, 53  # odd sizes: every row filter sees ragged rows
+    if mode == "I;16":
+        img = Image.fromarray(rng.integers(0, 65536, size=(h, w), dtype=np.uint16))
+    elif mode == "P":
+        img = Image.fromarray(rng.integers(0, 256, size=(h, w, 3), dtype=np.uint8)).quantize(colors=37)
+    else:
+        chans = {"L": 1, "LA": 2, "RGB": 3, "RGBA": 4}[mode]
+        arr = rng.integers(0, 256, size=(h, w, chans), dtype=np.uint8)
+        # smooth rows so the PNG encoder picks a mix of Sub / Up / Average / Paeth filters
+        arr[::2] = np.cumsum(arr[::2], axis=1, dtype=np.uint8)
+        img = Image.fromarray(arr[:, :, 0] if chans == 1 else arr)
+    assert img.mode == mode
+    src = tmp_path / ("hm_%s.png" % mode.replace(";", ""))
+    img.save(src, optimize=False)
+    out = tmp_path / "hm.bin"
+    subprocess.run([CHECK, "--png", str(src), str(out)], check=True)
+    raw = out.read_bytes()
+    W, H = np.frombuffer(raw[:8], dtype=np.uint32)
+    got = np.frombuffer(raw[8:], dtype=np.uint8).reshape(H, W)
+    assert (W, H) == (w, h)
+    assert np.array_equal(got, stb_like_red(Image.open(src)))
+
+
+def test_heightmap_png_loader_rejects_bad_files(tmp_path):
+    from PIL import Image
+    bad = tmp_path / "bad.png"
+    bad.write_bytes(b"not a png at all")
+    r = subprocess.run([CHECK, "--png", str(bad), str(tmp_path / "o.bin")], capture_output=True, text=True)
+    assert r.returncode != 0 and "not a PNG" in r.stderr
+    inter = tmp_path / "interlaced.png"
+    Image.fromarray(np.zeros((8, 8), np.uint8)).save(inter)
+    b = bytearray(inter.read_bytes())
+    b[28] = 1  # IHDR interlace method (Adam7), which the loader does not take
+    inter.write_bytes(bytes(b))
+    r = subprocess.run([CHECK, "--png", str(inter), str(tmp_path / "o.bin")], capture_output=True, text=True)
+    assert r.returncode != 0
+    trunc = tmp_path / "trunc.png"
+    Image.fromarray(np.zeros((8, 8), np.uint8)).save(trunc)
+    trunc.write_bytes(trunc.read_bytes()[:40])
+    r = subprocess.run([CHECK, "--png", str(trunc), str(tmp_path / "o.bin")], capture_output=True, text=True)
+    assert r.returncode != 0

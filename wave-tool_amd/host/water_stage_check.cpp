@@ -9,6 +9,9 @@
 //                                    then per frame position, normal, tangent, bitangent (f32)
 //                                    and the clip-space position (float4, f32).
 //   water_stage_check --indices <n>  prints the triangle index list of an n x n grid (no GPU)
+//   water_stage_check --png <in.png> <out.bin>
+//                                    decodes a heightmap PNG the way the reference loads it and
+//                                    writes width, height (u32) and the R8 texels (no GPU)
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -17,6 +20,7 @@
 #include <cstring>
 #include <vector>
 
+#include "heightmap_png.hpp"
 #include "water_stage.hpp"
 
 using wavetool::GerstnerWave;
@@ -36,7 +40,20 @@ int main(int argc, char** argv) {
     for (uint32_t v : WaterGrid::triangleIndices(n)) std::printf("%u\n", v);
     return 0;
   }
-  if (argc != 2) return fail("usage: water_stage_check <out.bin> | --indices <n>");
+  if (argc == 4 && std::strcmp(argv[1], "--png") == 0) {
+    std::vector<uint8_t> texels;
+    uint32_t w = 0, h = 0;
+    std::string err;
+    if (!wavetool::load_heightmap_png(argv[2], &texels, &w, &h, &err)) return fail("png:", err);
+    FILE* f = std::fopen(argv[3], "wb");
+    if (!f) return fail("cannot open", argv[3]);
+    std::fwrite(&w, 4, 1, f);
+    std::fwrite(&h, 4, 1, f);
+    std::fwrite(texels.data(), 1, texels.size(), f);
+    std::fclose(f);
+    return 0;
+  }
+  if (argc != 2) return fail("usage: water_stage_check <out.bin> | --indices <n> | --png <in.png> <out.bin>");
 
   WaveSet waves;
   const float d1 = 1.0f / std::sqrt(2.0f);
