@@ -165,6 +165,17 @@ wt_status wt_camera_matrices(const wt_camera* cam, double view[16], double proj[
  * WT_ERR_NO_INTERSECTION where the reference silently skips the water. */
 wt_status wt_projector_corners(const wt_camera* cam, float displaceable_amplitude, double corners[4][4]);
 
+/* The same projector (src/render-engine.cpp:797-1042) on the device for many
+ * cameras, one thread per camera, in double. d_cameras: count wt_camera structs;
+ * d_amplitudes: count slab half-heights. Writes d_corners[count][4][4] (NaN for a
+ * camera without corners) and d_status[count] (WT_OK, WT_ERR_NO_INTERSECTION,
+ * WT_ERR_DEGENERATE, or WT_ERR_INVALID_ARG for a camera wt_projector_corners would
+ * reject). All arrays are device memory; stream-ordered and capturable. Agrees with
+ * wt_projector_corners up to the rounding of the device's tan, sin, cos and acos. */
+#define WT_MAX_CAMERAS (1u << 24)
+wt_status wt_projector_corners_batched(wt_ctx* ctx, const wt_camera* d_cameras, const float* d_amplitudes,
+                                       uint32_t count, double* d_corners, int32_t* d_status, void* stream);
+
 /* Steepness normalisation and upload value, render-engine.cpp:1071-1073:
  * Q_i = (w * A != 0) ? q / (w * A * normaliser_count) : 0, written into
  * waves[i].steepness_Q_i. normaliser_count is GerstnerWave::Count() in the
@@ -253,6 +264,17 @@ wt_status wt_batch_vertex_count(const wt_batch* batch, uint64_t* vertices);
  * to minimise padded vertices; no reference counterpart). wt_set_launch_config with the
  * same shape makes wt_water_displace on that patch alone produce identical bits. */
 wt_status wt_batch_patch_launch_config(const wt_batch* batch, uint32_t patch, wt_launch_config* cfg);
+/* Re-aims every patch of a batch from its own camera on the device: patch i takes
+ * the corners of d_cameras[i] with slab half-height d_amplitudes[i] (as
+ * wt_projector_corners_batched) and cameraPosition = its position. A patch whose
+ * camera has no corners (status other than WT_OK, written to d_status[i] when
+ * d_status is not NULL) is skipped by later wt_water_displace_batched calls, which
+ * leave its outputs untouched, as the reference skips the water draw
+ * (src/render-engine.cpp:894), until a later call gives it corners again. Arrays
+ * hold one entry per patch in device memory; stream-ordered and capturable, so a
+ * captured frame loop moves cameras without a host round trip. */
+wt_status wt_batch_set_cameras(wt_ctx* ctx, wt_batch* batch, const wt_camera* d_cameras, const float* d_amplitudes,
+                               int32_t* d_status, void* stream);
 /* One persistent launch over every patch: each patch's vertex program at
  * time + time_offset. d_waves holds total_waves waves that the patches index into.
  * Replayable from a captured graph (the work counter resets itself). */

@@ -14,8 +14,8 @@
 #include <vector>
 
 #include "../../include/wt.h"
-#include "wt_host.hpp"
 #include "wt_kernels.hpp"
+#include "wt_projector.hpp"
 
 struct wt_ctx {
   int device = 0;
@@ -153,26 +153,11 @@ void wt_destroy(wt_ctx* ctx) { delete ctx; }
 
 const char* wt_last_error(const wt_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
-static wt::host::Camera make_camera(const wt_camera* c) {
-  wt::host::Camera cam(c->fov_deg, c->aspect, c->near_clip, c->far_clip,
-                       wt::host::Vec3{c->position[0], c->position[1], c->position[2]});
-  cam.set_rotation(c->yaw_deg, c->pitch_deg);
-  return cam;
-}
-
-static bool camera_ok(const wt_camera* c) {
-  if (!c) return false;
-  const float v[9] = {c->fov_deg, c->aspect, c->near_clip, c->far_clip, c->position[0],
-                      c->position[1], c->position[2], c->yaw_deg, c->pitch_deg};
-  for (float x : v)
-    if (!std::isfinite(x)) return false;
-  return c->fov_deg > 0.0f && c->fov_deg < 180.0f && c->aspect > 0.0f && c->near_clip > 0.0f &&
-         c->far_clip > c->near_clip;
-}
+static bool camera_ok(const wt_camera* c) { return c && wt::proj::camera_valid(*c); }
 
 wt_status wt_camera_matrices(const wt_camera* cam, double view[16], double proj[16], double forward[3]) {
   if (!camera_ok(cam) || !view || !proj || !forward) return WT_ERR_INVALID_ARG;
-  wt::host::Camera c = make_camera(cam);
+  const wt::proj::Camera c = wt::proj::camera_from(*cam);
   for (int i = 0; i < 16; ++i) {
     view[i] = c.view.c[i / 4][i % 4];
     proj[i] = c.proj.c[i / 4][i % 4];
@@ -185,11 +170,7 @@ wt_status wt_camera_matrices(const wt_camera* cam, double view[16], double proj[
 
 wt_status wt_projector_corners(const wt_camera* cam, float amp, double corners[4][4]) {
   if (!camera_ok(cam) || !corners || !std::isfinite(amp) || amp < 0.0f) return WT_ERR_INVALID_ARG;
-  wt::host::Camera c = make_camera(cam);
-  const int st = wt::host::projector_corners(c, (double)amp, corners);
-  if (st == wt::host::PROJ_NO_INTERSECTION) return WT_ERR_NO_INTERSECTION;
-  if (st != wt::host::PROJ_OK) return WT_ERR_DEGENERATE;
-  return WT_OK;
+  return (wt_status)wt::proj::projector_corners(wt::proj::camera_from(*cam), (double)amp, corners);
 }
 
 wt_status wt_normalize_steepness(wt_gerstner_wave* waves, const float* q, uint32_t count, uint32_t norm_count) {
@@ -235,15 +216,7 @@ wt_status wt_water_displace(wt_ctx* ctx, const wt_water_params* p, const wt_gers
   }
 
   wt::FwdParams fp{};
-  const double(*c)[4] = p->corners;  // [0] BL, [1] TL, [2] BR, [3] TR
-  fp.bl_x = c[0][0];
-  fp.bl_z = c[0][2];
-  fp.du_x = c[2][0] - c[0][0];
-  fp.du_z = c[2][2] - c[0][2];
-  fp.dv_x = c[1][0] - c[0][0];
-  fp.dv_z = c[1][2] - c[0][2];
-  fp.duv_x = c[3][0] - c[1][0] - c[2][0] + c[0][0];
-  fp.duv_z = c[3][2] - c[1][2] - c[2][2] + c[0][2];
+  wt::set_corner_algebra(fp, p->corners);  // [0] BL, [1] TL, [2] BR, [3] TR
   fp.inv_nm1 = 1.0 / (double)(N - 1);
   fp.time = p->waveAnimationTimeInSeconds;
   fp.N = (int)N;
@@ -541,12 +514,8 @@ wt_status wt_batch_create(wt_ctx* ctx, const wt_patch* patches, uint32_t count, 
       return fail(ctx, WT_ERR_INVALID_ARG, tag + "gridLength in [2, 32768], wave_count <= WT_MAX_WAVES, finite values");
     }
     wt::PatchDesc d{};
-    const double(*c)[4] = p.corners;
-    d.bl_x = c[0][0]; d.bl_z = c[0][2];
-    d.du_x = c[2][0] - c[0][0]; d.du_z = c[2][2] - c[0][2];
-    d.dv_x = c[1][0] - c[0][0]; d.dv_z = c[1][2] - c[0][2];
-    d.duv_x = c[3][0] - c[1][0] - c[2][0] + c[0][0];
-    d.duv_z = c[3][2] - c[1][2] - c[2][2] + c[0][2];
+    wt::set_corner_algebra(d, p.corners);
+    d.active = 1;
     d.inv_nm1 = 1.0 / (double)(p.gridLength - 1);
     d.time = p.waveAnimationTimeInSeconds;
     d.cam[0] = p.cameraPosition[0]; d.cam[1] = p.cameraPosition[1]; d.cam[2] = p.cameraPosition[2];
@@ -620,6 +589,57 @@ wt_status wt_batch_patch_launch_config(const wt_batch* b, uint32_t patch, wt_lau
   cfg->steps_per_lane = (uint32_t)s.steps;
   cfg->lanes_per_row = (uint32_t)s.lanes_per_row;
   cfg->wave_chunk = (uint32_t)s.wave_chunk;
+  return WT_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Shared checks of the two device-projector entry points: 4-byte aligned device arrays
+// (device memory is only checked outside stream capture, like the other entry points).
+wt_status projector_args(wt_ctx* ctx, const wt_camera* cams, const float* amps, const void* corners,
+                         const int32_t* status, hipStream_t st) {
+  if (!cams || !amps) return fail(ctx, WT_ERR_INVALID_ARG, "cameras and amplitudes are required");
+  const void* ptrs[4] = {cams, amps, corners, status};
+  for (const void* p : ptrs)
+    if (p && (reinterpret_cast<uintptr_t>(p) & 3)) return fail(ctx, WT_ERR_INVALID_ARG, "arrays must be 4-byte aligned");
+  if (corners && (reinterpret_cast<uintptr_t>(corners) & 7))
+    return fail(ctx, WT_ERR_INVALID_ARG, "corners must be 8-byte aligned");
+  if (!stream_capturing(st))
+    for (const void* p : ptrs)
+      if (p && !is_device_pointer(p)) return fail(ctx, WT_ERR_INVALID_ARG, "arrays must be device memory");
+  return WT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+wt_status wt_projector_corners_batched(wt_ctx* ctx, const wt_camera* d_cameras, const float* d_amplitudes,
+                                       uint32_t count, double* d_corners, int32_t* d_status, void* stream) {
+  if (!ctx) return WT_ERR_INVALID_ARG;
+  if (count == 0) return WT_OK;
+  if (count > WT_MAX_CAMERAS) return fail(ctx, WT_ERR_INVALID_ARG, "count exceeds WT_MAX_CAMERAS");
+  if (!d_corners || !d_status) return fail(ctx, WT_ERR_INVALID_ARG, "corners and status are required");
+  hipStream_t st = (hipStream_t)stream;
+  wt_status s = projector_args(ctx, d_cameras, d_amplitudes, d_corners, d_status, st);
+  if (s != WT_OK) return s;
+  wt::ProjectorArgs a{d_cameras, d_amplitudes, (int)count, d_corners, d_status, nullptr};
+  hipError_t e = wt::launch_projector(a, st);
+  if (e != hipSuccess) return hip_fail(ctx, e, "projector_kernel launch");
+  return WT_OK;
+}
+
+wt_status wt_batch_set_cameras(wt_ctx* ctx, wt_batch* b, const wt_camera* d_cameras, const float* d_amplitudes,
+                               int32_t* d_status, void* stream) {
+  if (!ctx || !b) return WT_ERR_INVALID_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  wt_status s = projector_args(ctx, d_cameras, d_amplitudes, nullptr, d_status, st);
+  if (s != WT_OK) return s;
+  wt::ProjectorArgs a{d_cameras, d_amplitudes, (int)b->patches.size(), nullptr, d_status, b->d_patches};
+  hipError_t e = wt::launch_projector(a, st);
+  if (e != hipSuccess) return hip_fail(ctx, e, "projector_kernel launch");
   return WT_OK;
 }
 

@@ -402,6 +402,7 @@ __global__ __launch_bounds__(256) void water_forward_batched_kernel(BatchArgs a)
     if (u >= a.n_units) break;
     const BatchUnit unit = a.units[u];
     const PatchDesc& pd = a.patches[unit.patch];
+    if (!pd.active) continue;  // workgroup-uniform: every thread takes the next unit
     const FwdParams p = patch_params(a, pd);
     switch (pd.shape) {
       case 0:

@@ -31,7 +31,8 @@ struct LaunchShape {
 
 hipError_t launch_forward(const FwdParams& p, const LaunchShape& s, hipStream_t st);
 
-// One patch of a batch on the device: corner algebra precomputed on the host.
+// One patch of a batch on the device: the corner algebra of FwdParams, set on the host
+// at batch creation or by the device projector (launch_projector).
 struct PatchDesc {
   double bl_x, bl_z, du_x, du_z, dv_x, dv_z, duv_x, duv_z, inv_nm1;
   double time;
@@ -40,7 +41,22 @@ struct PatchDesc {
   int N, W, wave_offset;
   int shape;                // segment shape of this patch in the batched kernel (kBatchShapes)
   long long vertex_offset;  // first vertex of the patch in the output arrays
+  int active;               // 0: the patch's camera misses the water; its units write nothing
+  int reserved;
 };
+
+// Corner algebra of a grid from its corners [4][4] (BL, TL, BR, TR; x and z used).
+template <class D>
+__host__ __device__ inline void set_corner_algebra(D& d, const double (*c)[4]) {
+  d.bl_x = c[0][0];
+  d.bl_z = c[0][2];
+  d.du_x = c[2][0] - c[0][0];
+  d.du_z = c[2][2] - c[0][2];
+  d.dv_x = c[1][0] - c[0][0];
+  d.dv_z = c[1][2] - c[0][2];
+  d.duv_x = c[3][0] - c[1][0] - c[2][0] + c[0][0];
+  d.duv_z = c[3][2] - c[1][2] - c[2][2] + c[0][2];
+}
 
 // One workgroup's segment group of one patch.
 struct BatchUnit {
@@ -93,6 +109,20 @@ constexpr LaunchShape kBatchShape[kBatchShapes] = {{2, 8, 64, 64}, {2, 4, 32, 64
 // The shape with the fewest padded vertex-waves for a grid of side N (a small premium
 // on the K = 4 shapes for their extra seeding).
 int batch_shape_for(int N);
+
+// Device projector: one thread per camera runs the reference projector in FP64.
+// corners (count x 16, row = corner BL, TL, BR, TR) and status (wt_status per camera)
+// may be null; with patches set, patch i takes camera i's corner algebra and camera
+// position and is switched off when its status is not WT_OK.
+struct ProjectorArgs {
+  const wt_camera* cameras;
+  const float* amplitudes;
+  int count;
+  double* corners;
+  int* status;
+  PatchDesc* patches;
+};
+hipError_t launch_projector(const ProjectorArgs& a, hipStream_t st);
 
 // Launches the persistent batched kernel with `grid` workgroups.
 hipError_t launch_forward_batched(const BatchArgs& a, int grid, hipStream_t st);

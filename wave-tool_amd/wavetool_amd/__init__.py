@@ -14,7 +14,7 @@ from ._abi import (LaunchConfig, Camera, GerstnerWave, Heightmap, VertexBuffers,
                    WT_WRAP_MIRRORED_REPEAT, WT_TEXEL_R8_UNORM, WT_TEXEL_R32F, WT_FLOP_PER_VERTEX_WAVE,
                    WT_BYTES_PER_VERTEX_PN, WT_BYTES_PER_VERTEX_PNTB)
 
-__all__ = ["Context", "camera", "projector_corners", "water_params", "normalize_steepness", "scenes", "lib", "check",
+__all__ = ["Context", "camera", "projector_corners", "projector_corners_batched", "camera_rows", "water_params", "normalize_steepness", "scenes", "lib", "check",
            "WtError"]
 
 
@@ -34,6 +34,27 @@ def projector_corners(cam_tuple, amp):
     st = lib().wt_projector_corners(ctypes.byref(camera(cam_tuple)), float(amp), out)
     check(st)
     return np.array(out[:], dtype=np.float64).reshape(4, 4)
+
+
+def camera_rows(cam_tuples):
+    """(count, 9) float32 array laid out as wt_camera structs, one row per camera tuple
+    (fov_deg, aspect, near, far, px, py, pz, yaw_deg, pitch_deg)."""
+    import numpy as np
+    return np.asarray([list(c) for c in cam_tuples], dtype=np.float32).reshape(-1, 9)
+
+
+def projector_corners_batched(ctx, cameras, amplitudes, corners, status, stream=None):
+    """Device projector over cameras (device float32 (count, 9), wt_camera rows) with
+    slab half-heights amplitudes (device float32 (count,)); fills corners (device float64
+    (count, 4, 4)) and status (device int32 (count,))."""
+    import torch
+    count = int(cameras.shape[0])
+    _check_array(cameras, torch.float32, count * 9, "cameras")
+    _check_array(amplitudes, torch.float32, count, "amplitudes")
+    _check_array(corners, torch.float64, count * 16, "corners")
+    _check_array(status, torch.int32, count, "status")
+    ctx._check(lib().wt_projector_corners_batched(ctx.handle, _ptr(cameras), _ptr(amplitudes), count, _ptr(corners),
+                                                  _ptr(status), _stream(stream)))
 
 
 def camera_matrices(cam_tuple):
@@ -74,6 +95,13 @@ def water_params(corners, cam_pos, N, time, sample_scale=0.0, displacement_scale
 
 def _ptr(t):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _check_array(t, dtype, numel, name):
+    """The kernels index count entries of each array: refuse a tensor that holds fewer."""
+    if t.dtype != dtype or not t.is_contiguous() or t.numel() != numel:
+        raise ValueError("%s: need a contiguous %s tensor of %d elements, got %s %s" %
+                         (name, dtype, numel, t.dtype, tuple(t.shape)))
 
 
 def _stream(stream):
@@ -211,6 +239,7 @@ class Batch:
 
     def __init__(self, ctx, patches):
         self.ctx = ctx
+        self.count = len(patches)
         arr = (Patch * len(patches))(*patches)
         h = ctypes.c_void_p()
         ctx._check(lib().wt_batch_create(ctx.handle, arr, len(patches), ctypes.byref(h)))
@@ -230,6 +259,18 @@ class Batch:
                                              ctypes.byref(hm) if hm is not None else None, ctypes.byref(vb),
                                              float(time_offset), _stream(stream))
         self.ctx._check(st)
+
+    def set_cameras(self, cameras, amplitudes, status=None, stream=None):
+        """Re-aim every patch from its own camera on the device (wt_batch_set_cameras):
+        cameras device float32 (patches, 9), amplitudes device float32 (patches,),
+        status optional device int32 (patches,)."""
+        import torch
+        _check_array(cameras, torch.float32, self.count * 9, "cameras")
+        _check_array(amplitudes, torch.float32, self.count, "amplitudes")
+        if status is not None:
+            _check_array(status, torch.int32, self.count, "status")
+        self.ctx._check(lib().wt_batch_set_cameras(self.ctx.handle, self.handle, _ptr(cameras), _ptr(amplitudes),
+                                                   _ptr(status), _stream(stream)))
 
     def patch_launch_config(self, i):
         """(wave_split, steps, lanes_per_row, wave_chunk) the batch runs patch i with."""

@@ -155,6 +155,7 @@ PROTOTYPES = [
     ("wt_last_error", ctypes.c_char_p, [_vp]),
     ("wt_camera_matrices", ctypes.c_int, [ctypes.POINTER(Camera), _dp, _dp, _dp]),
     ("wt_projector_corners", ctypes.c_int, [ctypes.POINTER(Camera), ctypes.c_float, _dp]),
+    ("wt_projector_corners_batched", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_uint32, _vp, _vp, _vp]),
     ("wt_normalize_steepness", ctypes.c_int,
      [ctypes.POINTER(GerstnerWave), ctypes.POINTER(ctypes.c_float), ctypes.c_uint32, ctypes.c_uint32]),
     ("wt_water_displace", ctypes.c_int,
@@ -178,6 +179,7 @@ PROTOTYPES = [
     ("wt_batch_destroy", None, [_vp]),
     ("wt_batch_vertex_count", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64)]),
     ("wt_batch_patch_launch_config", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.POINTER(LaunchConfig)]),
+    ("wt_batch_set_cameras", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     ("wt_water_displace_batched", ctypes.c_int,
      [_vp, _vp, _vp, ctypes.c_uint32, ctypes.POINTER(Heightmap), ctypes.POINTER(VertexBuffers), ctypes.c_double, _vp]),
     ("wt_fit_create", ctypes.c_int, [_vp, ctypes.POINTER(FitProblem), ctypes.c_uint32, ctypes.POINTER(_vp)]),
