@@ -85,15 +85,21 @@ WT_HD bool mat_inverse(const Mat4& m, Mat4* out) {
     }
   for (int p = 0; p < 4; ++p) {
     int best = p;
+    double big = fabs(a[p][p]);
     for (int r = p + 1; r < 4; ++r)
-      if (fabs(a[r][p]) > fabs(a[best][p])) best = r;
-    if (a[best][p] == 0.0) return false;
-    if (best != p)
-      for (int k = 0; k < 8; ++k) {
-        const double t = a[p][k];
-        a[p][k] = a[best][k];
-        a[best][k] = t;
+      if (fabs(a[r][p]) > big) {
+        best = r;
+        big = fabs(a[r][p]);
       }
+    if (big == 0.0) return false;
+    // row swap written with constant row indices so the device keeps `a` in registers
+    for (int r = p + 1; r < 4; ++r)
+      if (r == best)
+        for (int k = 0; k < 8; ++k) {
+          const double t = a[p][k];
+          a[p][k] = a[r][k];
+          a[r][k] = t;
+        }
     const double inv = 1.0 / a[p][p];
     for (int k = 0; k < 8; ++k) a[p][k] *= inv;
     for (int r = 0; r < 4; ++r) {
@@ -291,6 +297,7 @@ WT_HD int projector_corners(const Camera& cam, double amp, double corners[4][4])
   const int edges[12][2] = {{0, 1}, {0, 2}, {0, 4}, {1, 3}, {1, 5}, {2, 3},
                             {2, 6}, {3, 7}, {4, 5}, {4, 6}, {5, 7}, {6, 7}};
   NdcBounds b;
+#pragma unroll
   for (int e = 0; e < 12; ++e) {
     const Vec3 &p = fc[edges[e][0]], &q = fc[edges[e][1]];
     const double lo = fmin(p.y, q.y), hi = fmax(p.y, q.y);
