@@ -233,6 +233,15 @@ wt_status wt_autotune_forward(wt_ctx* ctx, uint32_t grid_length, uint32_t wave_c
 wt_status wt_autotune_save(wt_ctx* ctx, const char* path);
 wt_status wt_autotune_load(wt_ctx* ctx, const char* path);
 
+/* The water grid's index buffer (src/program.cpp:686-717) written on the device: for
+ * each cell (r, c) of the gridLength x gridLength grid, row-major, the triangles
+ * (r,c) (r+1,c+1) (r+1,c) and (r,c) (r,c+1) (r+1,c+1), counter-clockwise seen from
+ * +y, with vertex id = row * gridLength + col + base_vertex (base_vertex = a patch's
+ * vertex_offset for a batch's shared vertex arrays). d_indices holds
+ * 6 (gridLength - 1)^2 uint32 in device memory. Stream-ordered and capturable. */
+wt_status wt_grid_triangle_indices(wt_ctx* ctx, uint32_t grid_length, uint32_t base_vertex, uint32_t* d_indices,
+                                   void* stream);
+
 /* ---------------------------------------------------------------- batched forward */
 /* One ocean patch of a batch: its own projected grid (corners from
  * wt_projector_corners of its own camera), time, scales and a run of the batch's

@@ -68,3 +68,23 @@ def compare(out, ref, cam_pos, edge_mask=None, radius=RADIUS):
         if out.get(k) is not None and ref.get(k) is not None:
             res[k + "_max"] = float(angle_between(out[k].astype(np.float64), ref[k])[nmask].max())
     return res
+
+
+def reference_indices(n):
+    """src/program.cpp:686-717 restated: ids row-major from the bottom-left, per cell
+    (r,c), (r+1,c+1), (r+1,c) then (r,c), (r,c+1), (r+1,c+1)."""
+    ids = np.arange(n * n).reshape(n, n)
+    out = []
+    for r in range(n - 1):
+        for c in range(n - 1):
+            out += [ids[r, c], ids[r + 1, c + 1], ids[r + 1, c], ids[r, c], ids[r, c + 1], ids[r + 1, c + 1]]
+    return np.array(out, dtype=np.int64)
+
+
+def reference_indices_vectorized(n):
+    """The same list built with array operations, for large n (pinned against the loop
+    form on small grids)."""
+    r, c = np.meshgrid(np.arange(n - 1, dtype=np.int64), np.arange(n - 1, dtype=np.int64), indexing="ij")
+    bl = r * n + c
+    tl = bl + n
+    return np.stack([bl, tl + 1, tl, bl, bl + 1, tl + 1], axis=-1).reshape(-1)

@@ -643,6 +643,22 @@ wt_status wt_batch_set_cameras(wt_ctx* ctx, wt_batch* b, const wt_camera* d_came
   return WT_OK;
 }
 
+wt_status wt_grid_triangle_indices(wt_ctx* ctx, uint32_t grid_length, uint32_t base_vertex, uint32_t* d_indices,
+                                   void* stream) {
+  if (!ctx) return WT_ERR_INVALID_ARG;
+  if (grid_length < 2 || grid_length > 32768) return fail(ctx, WT_ERR_INVALID_ARG, "gridLength must be in [2, 32768]");
+  if ((uint64_t)base_vertex + (uint64_t)grid_length * grid_length > (1ull << 32))
+    return fail(ctx, WT_ERR_INVALID_ARG, "base_vertex + gridLength^2 exceeds the 32-bit index range");
+  if (!d_indices || (reinterpret_cast<uintptr_t>(d_indices) & 3))
+    return fail(ctx, WT_ERR_INVALID_ARG, "indices must be a 4-byte aligned device array");
+  hipStream_t st = (hipStream_t)stream;
+  if (!stream_capturing(st) && !is_device_pointer(d_indices))
+    return fail(ctx, WT_ERR_INVALID_ARG, "indices must be device memory");
+  hipError_t e = wt::launch_grid_indices(grid_length, base_vertex, d_indices, st);
+  if (e != hipSuccess) return hip_fail(ctx, e, "grid_indices_kernel launch");
+  return WT_OK;
+}
+
 wt_status wt_water_displace_batched(wt_ctx* ctx, wt_batch* b, const wt_gerstner_wave* d_waves, uint32_t total_waves,
                                     const wt_heightmap* hm, const wt_vertex_buffers* out, double time_offset,
                                     void* stream) {

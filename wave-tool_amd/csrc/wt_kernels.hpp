@@ -124,6 +124,9 @@ struct ProjectorArgs {
 };
 hipError_t launch_projector(const ProjectorArgs& a, hipStream_t st);
 
+// Triangle index list of an n x n grid (n >= 2), 6 (n - 1)^2 entries, ids offset by base.
+hipError_t launch_grid_indices(uint32_t n, uint32_t base, uint32_t* out, hipStream_t st);
+
 // Launches the persistent batched kernel with `grid` workgroups.
 hipError_t launch_forward_batched(const BatchArgs& a, int grid, hipStream_t st);
 // Workgroups of the batched kernel resident per CU.

@@ -133,6 +133,13 @@ class Context:
     def _check(self, st):
         check(st, self.handle)
 
+    def grid_indices(self, n, out, base=0, stream=None):
+        """Write the triangle index list of an n x n grid (ids + base) into the device int32
+        tensor `out` of 6 (n - 1)^2 elements (read it as uint32)."""
+        import torch
+        _check_array(out, torch.int32, 6 * (int(n) - 1) ** 2, "indices")
+        self._check(lib().wt_grid_triangle_indices(self.handle, int(n), int(base), _ptr(out), _stream(stream)))
+
     def displace(self, params, waves, pos, nrm, tan=None, bit=None, heightmap=None, hm_format=WT_TEXEL_R32F,
                  hm_wrap=WT_WRAP_REPEAT, stream=None, wave_count=None):
         """waves: device tensor (W, 8) float32; pos/nrm/tan/bit: device float tensors

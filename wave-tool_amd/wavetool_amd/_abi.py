@@ -180,6 +180,7 @@ PROTOTYPES = [
     ("wt_batch_vertex_count", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64)]),
     ("wt_batch_patch_launch_config", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.POINTER(LaunchConfig)]),
     ("wt_batch_set_cameras", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    ("wt_grid_triangle_indices", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp]),
     ("wt_water_displace_batched", ctypes.c_int,
      [_vp, _vp, _vp, ctypes.c_uint32, ctypes.POINTER(Heightmap), ctypes.POINTER(VertexBuffers), ctypes.c_double, _vp]),
     ("wt_fit_create", ctypes.c_int, [_vp, ctypes.POINTER(FitProblem), ctypes.c_uint32, ctypes.POINTER(_vp)]),
