@@ -134,6 +134,47 @@ def test_small_and_empty(torch_cuda, wtctx):
             assert res["pos_max"] <= POS_TOL and res["nrm_max"] <= NRM_TOL
 
 
+def test_max_wave_count(torch_cuda, wtctx):
+    """WT_MAX_WAVES = 1024 waves (16 LDS chunks) with the heightmap, steepness normalised by
+    the count; amplitudes / 16 so the sum stays at ocean scale."""
+    say("N=48 W=1024 heightmap")
+    cam = scenes.camera_tuple()
+    corners = wt.projector_corners(cam, 2.0)
+    waves = scenes.gen_waves(51, 1024).copy()
+    waves[:, 0] /= 16.0
+    cfg = dict(N=48, sample_scale=0.02, displacement_scale=0.05, scroll=(0.3, 0.0))
+    res = check_parity(torch_cuda, wtctx, cfg, corners, waves, cam[4:7], 10.0, scenes.value_noise(5, 64), True)
+    assert res["pos_max"] <= POS_TOL and res["nrm_max"] <= NRM_TOL
+
+
+def test_largest_grid(torch_cuda, wtctx):
+    """gridLength 32768, the ABI's limit: 1.07e9 vertices, 12.9 GB per output array.
+    The bottom, middle and top rows against the oracle."""
+    torch = torch_cuda
+    N, W = 32768, 4
+    say("N=%d W=%d (two 12.9 GB outputs)" % (N, W))
+    cam = scenes.camera_tuple()
+    corners = wt.projector_corners(cam, 2.0)
+    waves_np = scenes.gen_waves(61, W)
+    params = wt.water_params(corners, cam[4:7], N, 10.0)
+    pos = torch.empty(N * N * 3, dtype=torch.float32, device="cuda")
+    nrm = torch.empty_like(pos)
+    wtctx.displace(params, torch.from_numpy(waves_np).cuda(), pos, nrm)
+    torch.cuda.synchronize()
+    sd = oracle_scene(corners, cam[4:7], N, waves_np, 10.0)
+    for row in (0, N // 2, N - 1):
+        ref = orc.vertices(sd, first=row * N, count=N, nthreads=NTHREADS, want=("pos", "nrm"))
+        lo, hi = 3 * row * N, 3 * (row + 1) * N
+        got = dict(pos=pos[lo:hi].cpu().numpy().reshape(N, 3), nrm=nrm[lo:hi].cpu().numpy().reshape(N, 3))
+        res = compare(got, ref, cam[4:7])
+        say("row %d: %s" % (row, res))
+        assert row != 0 or res["n_near"] == N
+        assert np.isfinite(got["pos"]).all() and np.isfinite(got["nrm"]).all()
+        assert res["pos_max"] <= POS_TOL and res["nrm_max"] <= NRM_TOL
+    del pos, nrm
+    torch.cuda.empty_cache()
+
+
 def test_zero_waves_is_flat(torch_cuda, wtctx):
     """W = 0 and no heightmap: positions are the bilinear grid points on y = bounce and
     every normal is +y (camera above the plane)."""
