@@ -7,10 +7,11 @@
 // Loss = 1 / (2 S V) * sum over snapshots and grid points of |d - target|^2.
 //
 // adjoint_kernel<SMAX, KP>: KP grid points per thread (256 KP per block), all S <= SMAX
-//   snapshots of each in registers; one instantiation per snapshot class (8, 16, 32),
-//   so a batch of mixed problems runs one launch per class present.
+//   snapshots of each in registers; one instantiation per snapshot class (SMAX = 4, 8,
+//   ..., 32), so a batch of mixed problems runs one launch per class present.
 //   forward: per wave, seed (cos, sin) from the FP64-reduced phase at t0 and step the
-//     snapshots with a rotation by omega dt; accumulate d per snapshot.
+//     snapshots with the three-term recurrence (or, for |sin(omega dt)| < 0.25, a
+//     rotation by omega dt); accumulate d per snapshot.
 //   residual r = d - target, loss contribution 1/2 |r|^2.
 //   backward: per wave, re-seed and step again (recomputing the phases is cheaper than
 //     storing them: 4 VALU ops per snapshot against an 8-byte HBM round trip), with six
@@ -36,8 +37,17 @@ struct FitWaveLds {
   double om_rev;       // omega / 2 pi, revolutions per second
   float qadx, qadz, A, Q, Dx, Dz;
   float rc, rs;        // rotation by omega dt
+  float tc;            // 2 cos(omega dt), the three-term recurrence's coefficient
+  int orig;            // the wave's index in the problem (its gradient rows)
   float w, wL, om2L;   // w, w / L, omega / (2 L)
 };
+
+// The three-term recurrence x[s+1] = 2 cos(d) x[s] - x[s-1] steps (cos, sin) with one FMA
+// per value instead of the rotation's two. Its FP32 errors grow by at most 1 / |sin d|
+// per step and the rounding of 2 cos d shifts the step by at most 1.2e-7 / (2 |sin d|),
+// so with |sin d| >= 0.25 and at most 32 snapshots the phases stay within 1e-5 rad;
+// waves with a smaller |sin d| keep the rotation.
+constexpr double kFitChebMinSin = 0.25;
 
 // Sum over the 64 lanes of a wavefront with DPP moves (VALU only, fixed order); the
 // total lands in lane 63. quad_perm [1,0,3,2] and [2,3,0,1] give quad sums,
@@ -56,6 +66,19 @@ __device__ __forceinline__ float wave_sum63(float v) {
   return v;
 }
 
+// The snapshot step omega dt of wave i in radians, reduced to [0, 2 pi).
+__device__ __forceinline__ double fit_wave_step(const FitDesc& P, int i) {
+  const float* pr = P.params + 5 * i;
+  const double L = pr[1] > 1e-3f ? pr[1] : 1e-3f;
+  const double om = sqrt(P.g * (kTwoPi / L));
+  const double step = om * P.dt * kInvTwoPi;
+  return (step - floor(step)) * kTwoPi;
+}
+
+__device__ __forceinline__ int fit_wave_cheb(const FitDesc& P, int i) {
+  return fabs(sin(fit_wave_step(P, i))) >= kFitChebMinSin;
+}
+
 __device__ __forceinline__ void stage_fit_wave(const FitDesc& P, int i, FitWaveLds* o) {
   const float* pr = P.params + 5 * i;
   const double A = pr[0], L = pr[1] > 1e-3f ? pr[1] : 1e-3f, al = pr[2], Q = pr[3], ph0 = pr[4];
@@ -67,12 +90,12 @@ __device__ __forceinline__ void stage_fit_wave(const FitDesc& P, int i, FitWaveL
   const double ph = (om * P.t0 + ph0) * kInvTwoPi;
   o->c0 = ph - floor(ph);
   o->om_rev = om * kInvTwoPi;
-  const double step = om * P.dt * kInvTwoPi;
-  const double ang = (step - floor(step)) * kTwoPi;
   double sn, cs;
-  sincos(ang, &sn, &cs);
+  sincos(fit_wave_step(P, i), &sn, &cs);
   o->rc = (float)cs;
   o->rs = (float)sn;
+  o->tc = (float)(2.0 * cs);
+  o->orig = i;
   o->A = (float)A;
   o->Q = (float)Q;
   o->Dx = (float)ca;
@@ -84,8 +107,116 @@ __device__ __forceinline__ void stage_fit_wave(const FitDesc& P, int i, FitWaveL
   o->om2L = (float)(om / (2.0 * L));
 }
 
+// (c, sn) <- the next snapshot's (cos, sin). Rotation form: by omega dt. Three-term form:
+// a rotation from snapshot 0, which also sets the previous values (cp, sp), then
+// x[s+1] = 2 cos(omega dt) x[s] - x[s-1].
+template <bool kCheb>
+__device__ __forceinline__ void step_snapshot(int s, float& c, float& sn, float& cp, float& sp, const FitWaveLds& wv) {
+  float cn, snn;
+  if (!kCheb || s == 0) {
+    cn = fmaf(sn, -wv.rs, c * wv.rc);
+    snn = fmaf(c, wv.rs, sn * wv.rc);
+  } else {
+    cn = fmaf(wv.tc, c, -cp);
+    snn = fmaf(wv.tc, sn, -sp);
+  }
+  if (kCheb) {
+    cp = c;
+    sp = sn;
+  }
+  c = cn;
+  sn = snn;
+}
+
+// Forward of one wave at the thread's points: add its displacement to every snapshot.
+template <int SMAX, int KP, bool kCheb>
+__device__ __forceinline__ void forward_wave(const FitWaveLds& wv, int S, const double (&x)[KP],
+                                             const double (&z)[KP], float (&r)[KP][SMAX][3]) {
+#pragma unroll
+  for (int k = 0; k < KP; ++k) {
+    const float ph = (float)__builtin_amdgcn_fract(fma(wv.kx, x[k], fma(wv.kz, z[k], wv.c0)));
+    float c = __builtin_amdgcn_cosf(ph), sn = __builtin_amdgcn_sinf(ph), cp = 0.0f, sp = 0.0f;
+#pragma unroll
+    for (int s = 0; s < SMAX; ++s) {
+      r[k][s][0] = fmaf(wv.qadx, c, r[k][s][0]);
+      r[k][s][1] = fmaf(wv.A, sn, r[k][s][1]);
+      r[k][s][2] = fmaf(wv.qadz, c, r[k][s][2]);
+      step_snapshot<kCheb>(s, c, sn, cp, sp, wv);
+    }
+  }
+}
+
+// Backward of one wave: six sums over snapshots and three sums of their running values
+// per point, then the five derivatives, added over the thread's points, reduced over
+// the wavefront and written to the wave's rows of s_red.
+template <int SMAX, int KP, bool kCheb>
+__device__ __forceinline__ void backward_wave(const FitDesc& P, const FitWaveLds& wv, int S, const double (&x)[KP],
+                                              const double (&z)[KP], const float (&r)[KP][SMAX][3], float* s_red,
+                                              int lane, int wid) {
+  const float x0f = (float)P.x0, z0f = (float)P.z0;
+  const float dtf = (float)P.dt, Sf = (float)SMAX, t0f = (float)P.t0;
+  const float A = wv.A, Q = wv.Q, Dx = wv.Dx, Dz = wv.Dz;
+  const float origin_dot = fmaf(Dx, x0f, Dz * z0f), origin_perp = fmaf(-Dz, x0f, Dx * z0f);
+  float gA = 0.0f, gL = 0.0f, gAl = 0.0f, gQ = 0.0f, gP = 0.0f;
+#pragma unroll
+  for (int k = 0; k < KP; ++k) {
+    const float ph = (float)__builtin_amdgcn_fract(fma(wv.kx, x[k], fma(wv.kz, z[k], wv.c0)));
+    float c = __builtin_amdgcn_cosf(ph), sn = __builtin_amdgcn_sinf(ph), cp = 0.0f, sp = 0.0f;
+    float sxc = 0, szc = 0, syc = 0, sxs = 0, szs = 0, sys = 0, vyc = 0, vxs = 0, vzs = 0;
+#pragma unroll
+    for (int s = 0; s < SMAX; ++s) {
+      const float rx = r[k][s][0], ry = r[k][s][1], rz = r[k][s][2];
+      sxc = fmaf(rx, c, sxc);
+      szc = fmaf(rz, c, szc);
+      syc = fmaf(ry, c, syc);
+      sxs = fmaf(rx, sn, sxs);
+      szs = fmaf(rz, sn, szs);
+      sys = fmaf(ry, sn, sys);
+      vyc += syc;
+      vxs += sxs;
+      vzs += szs;
+      step_snapshot<kCheb>(s, c, sn, cp, sp, wv);
+    }
+    // time-weighted sums: sum_s (s dt) f_s = dt (SMAX U - V), the residuals of the
+    // snapshots past S being zero
+    const float tsyc = dtf * fmaf(Sf, syc, -vyc);
+    const float tsxs = dtf * fmaf(Sf, sxs, -vxs);
+    const float tszs = dtf * fmaf(Sf, szs, -vzs);
+    const float sec = Dx * sxc + Dz * szc;    // sum of (r . D) cos
+    const float sepc = -Dz * sxc + Dx * szc;  // sum of (r . D-perp) cos
+    const float g0 = A * (syc - Q * (Dx * sxs + Dz * szs));     // sum of r . dd/dtheta
+    const float gt = A * (tsyc - Q * (Dx * tsxs + Dz * tszs));  // sum of (t - t0) r . dd/dtheta
+    const float g0t = fmaf(t0f, g0, gt);                        // sum of t r . dd/dtheta
+    // D.x and D-perp.x split into the problem origin term and the local term
+    const float xf = (float)(x[k] - P.x0), zf = (float)(z[k] - P.z0);
+    const float dotx = fmaf(Dx, xf, Dz * zf) + origin_dot;
+    const float perpx = fmaf(-Dz, xf, Dx * zf) + origin_perp;
+    // inactive points have zero residuals, so every sum above is zero for them
+    gA += fmaf(Q, sec, sys);
+    gL += -wv.wL * dotx * g0 - wv.om2L * g0t;
+    gAl += fmaf(wv.w * perpx, g0, Q * A * sepc);
+    gQ += A * sec;
+    gP += g0;
+  }
+  gA = wave_sum63(gA);
+  gL = wave_sum63(gL);
+  gAl = wave_sum63(gAl);
+  gQ = wave_sum63(gQ);
+  gP = wave_sum63(gP);
+  if (lane == 63) {
+    float* o = s_red + (wv.orig * 5) * 4 + wid;
+    o[0 * 4] = gA;
+    o[1 * 4] = gL;
+    o[2 * 4] = gAl;
+    o[3 * 4] = gQ;
+    o[4 * 4] = gP;
+  }
+}
+
+// s_wave holds the problem's waves with the ncheb three-term-recurrence waves first, so
+// each loop over waves is branch-free.
 template <int SMAX, int KP>
-__device__ __forceinline__ void adjoint_body(const FitDesc& P, int local_block, const FitWaveLds* s_wave,
+__device__ __forceinline__ void adjoint_body(const FitDesc& P, int local_block, const FitWaveLds* s_wave, int ncheb,
                                              float* s_red) {
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int V = P.n * P.n;
@@ -109,114 +240,51 @@ __device__ __forceinline__ void adjoint_body(const FitDesc& P, int local_block, 
     for (int s = 0; s < SMAX; ++s) r[k][s][0] = r[k][s][1] = r[k][s][2] = 0.0f;
 
   // forward: d per snapshot
-  for (int i = 0; i < P.W; ++i) {
-    const FitWaveLds& wv = s_wave[i];
-    const float qadx = wv.qadx, qadz = wv.qadz, A = wv.A, rc = wv.rc, rs = wv.rs, nrs = -wv.rs;
-#pragma unroll
-    for (int k = 0; k < KP; ++k) {
-      const float ph = (float)__builtin_amdgcn_fract(fma(wv.kx, x[k], fma(wv.kz, z[k], wv.c0)));
-      float c = __builtin_amdgcn_cosf(ph), sn = __builtin_amdgcn_sinf(ph);
-#pragma unroll
-      for (int s = 0; s < SMAX; ++s) {
-        if (s < S) {
-          r[k][s][0] = fmaf(qadx, c, r[k][s][0]);
-          r[k][s][1] = fmaf(A, sn, r[k][s][1]);
-          r[k][s][2] = fmaf(qadz, c, r[k][s][2]);
-          const float cn = fmaf(sn, nrs, c * rc);
-          sn = fmaf(c, rs, sn * rc);
-          c = cn;
-        }
-      }
-    }
-  }
+  int i = 0;
+  for (; i < ncheb; ++i) forward_wave<SMAX, KP, true>(s_wave[i], S, x, z, r);
+  for (; i < P.W; ++i) forward_wave<SMAX, KP, false>(s_wave[i], S, x, z, r);
   // residuals and loss
   float loss = 0.0f;
 #pragma unroll
   for (int k = 0; k < KP; ++k) {
 #pragma unroll
     for (int s = 0; s < SMAX; ++s) {
-      if (s < S) {
-        const float* tg = P.target + ((size_t)s * V + vv[k]) * 3;
-        const float t0 = tg[0], t1 = tg[1], t2 = tg[2];
-        r[k][s][0] = active[k] ? r[k][s][0] - t0 : 0.0f;
-        r[k][s][1] = active[k] ? r[k][s][1] - t1 : 0.0f;
-        r[k][s][2] = active[k] ? r[k][s][2] - t2 : 0.0f;
-        loss = fmaf(0.5f * r[k][s][0], r[k][s][0], loss);
-        loss = fmaf(0.5f * r[k][s][1], r[k][s][1], loss);
-        loss = fmaf(0.5f * r[k][s][2], r[k][s][2], loss);
-      }
+      // snapshots past S (and points past the grid) get zero residuals; their loads
+      // read the last snapshot so every address stays inside the target
+      const bool use = active[k] && s < S;
+      const float* tg = P.target + ((size_t)(s < S ? s : S - 1) * V + vv[k]) * 3;
+      const float t0 = tg[0], t1 = tg[1], t2 = tg[2];
+      r[k][s][0] = use ? r[k][s][0] - t0 : 0.0f;
+      r[k][s][1] = use ? r[k][s][1] - t1 : 0.0f;
+      r[k][s][2] = use ? r[k][s][2] - t2 : 0.0f;
+      loss = fmaf(0.5f * r[k][s][0], r[k][s][0], loss);
+      loss = fmaf(0.5f * r[k][s][1], r[k][s][1], loss);
+      loss = fmaf(0.5f * r[k][s][2], r[k][s][2], loss);
     }
   }
   const float lsum = wave_sum63(loss);
   if (lane == 63) s_red[(P.W * 5) * 4 + wid] = lsum;
 
-  // backward: per wave, six sums over snapshots and three sums of their running values
-  // per point, then the five derivatives, added over the thread's points
-  const float x0f = (float)P.x0, z0f = (float)P.z0;
-  const float dtf = (float)P.dt, Sf = (float)S, t0f = (float)P.t0;
-  for (int i = 0; i < P.W; ++i) {
-    const FitWaveLds& wv = s_wave[i];
-    const float rc = wv.rc, rs = wv.rs, nrs = -wv.rs;
-    const float A = wv.A, Q = wv.Q, Dx = wv.Dx, Dz = wv.Dz;
-    const float origin_dot = fmaf(Dx, x0f, Dz * z0f), origin_perp = fmaf(-Dz, x0f, Dx * z0f);
-    float gA = 0.0f, gL = 0.0f, gAl = 0.0f, gQ = 0.0f, gP = 0.0f;
-#pragma unroll
-    for (int k = 0; k < KP; ++k) {
-      const float ph = (float)__builtin_amdgcn_fract(fma(wv.kx, x[k], fma(wv.kz, z[k], wv.c0)));
-      float c = __builtin_amdgcn_cosf(ph), sn = __builtin_amdgcn_sinf(ph);
-      float sxc = 0, szc = 0, syc = 0, sxs = 0, szs = 0, sys = 0, vyc = 0, vxs = 0, vzs = 0;
-#pragma unroll
-      for (int s = 0; s < SMAX; ++s) {
-        if (s < S) {
-          const float rx = r[k][s][0], ry = r[k][s][1], rz = r[k][s][2];
-          sxc = fmaf(rx, c, sxc);
-          szc = fmaf(rz, c, szc);
-          syc = fmaf(ry, c, syc);
-          sxs = fmaf(rx, sn, sxs);
-          szs = fmaf(rz, sn, szs);
-          sys = fmaf(ry, sn, sys);
-          vyc += syc;
-          vxs += sxs;
-          vzs += szs;
-          const float cn = fmaf(sn, nrs, c * rc);
-          sn = fmaf(c, rs, sn * rc);
-          c = cn;
-        }
-      }
-      // time-weighted sums: sum_s (s dt) f_s = dt (S U - V)
-      const float tsyc = dtf * fmaf(Sf, syc, -vyc);
-      const float tsxs = dtf * fmaf(Sf, sxs, -vxs);
-      const float tszs = dtf * fmaf(Sf, szs, -vzs);
-      const float sec = Dx * sxc + Dz * szc;    // sum of (r . D) cos
-      const float sepc = -Dz * sxc + Dx * szc;  // sum of (r . D-perp) cos
-      const float g0 = A * (syc - Q * (Dx * sxs + Dz * szs));     // sum of r . dd/dtheta
-      const float gt = A * (tsyc - Q * (Dx * tsxs + Dz * tszs));  // sum of (t - t0) r . dd/dtheta
-      const float g0t = fmaf(t0f, g0, gt);                        // sum of t r . dd/dtheta
-      // D.x and D-perp.x split into the problem origin term and the local term
-      const float xf = (float)(x[k] - P.x0), zf = (float)(z[k] - P.z0);
-      const float dotx = fmaf(Dx, xf, Dz * zf) + origin_dot;
-      const float perpx = fmaf(-Dz, xf, Dx * zf) + origin_perp;
-      // inactive points have zero residuals, so every sum above is zero for them
-      gA += fmaf(Q, sec, sys);
-      gL += -wv.wL * dotx * g0 - wv.om2L * g0t;
-      gAl += fmaf(wv.w * perpx, g0, Q * A * sepc);
-      gQ += A * sec;
-      gP += g0;
-    }
-    gA = wave_sum63(gA);
-    gL = wave_sum63(gL);
-    gAl = wave_sum63(gAl);
-    gQ = wave_sum63(gQ);
-    gP = wave_sum63(gP);
-    if (lane == 63) {
-      float* o = s_red + (i * 5) * 4 + wid;
-      o[0 * 4] = gA;
-      o[1 * 4] = gL;
-      o[2 * 4] = gAl;
-      o[3 * 4] = gQ;
-      o[4 * 4] = gP;
-    }
+  // backward
+  for (i = 0; i < ncheb; ++i) backward_wave<SMAX, KP, true>(P, s_wave[i], S, x, z, r, s_red, lane, wid);
+  for (; i < P.W; ++i) backward_wave<SMAX, KP, false>(P, s_wave[i], S, x, z, r, s_red, lane, wid);
+}
+
+// Stage the problem's waves into LDS, three-term-recurrence waves first (each group in
+// wave order); returns their number. Every thread of the block calls it.
+__device__ __forceinline__ int stage_fit_waves(const FitDesc& P, FitWaveLds* s_wave, int* s_flag) {
+  for (int i = threadIdx.x; i < P.W; i += blockDim.x) s_flag[i] = fit_wave_cheb(P, i);
+  __syncthreads();
+  int ncheb = 0;
+  for (int j = 0; j < P.W; ++j) ncheb += s_flag[j];
+  for (int i = threadIdx.x; i < P.W; i += blockDim.x) {
+    int before = 0;
+    for (int j = 0; j < i; ++j) before += s_flag[j];
+    const int pos = s_flag[i] ? before : ncheb + (i - before);
+    stage_fit_wave(P, i, &s_wave[pos]);
   }
+  __syncthreads();
+  return ncheb;
 }
 
 // One launch covers the blocks [base, base + gridDim.x) of the batch's block -> problem
@@ -227,13 +295,13 @@ __global__ __launch_bounds__(kFitThreads) void adjoint_kernel(const FitDesc* __r
                                                               int bump_step) {
   __shared__ FitWaveLds s_wave[kFitMaxWaves];
   __shared__ float s_red[(kFitMaxWaves * 5 + 1) * 4];
+  __shared__ int s_flag[kFitMaxWaves];
   const long long gb = base + blockIdx.x;
   const FitDesc P = probs[block_prob[gb]];
   const int local_block = (int)(gb - P.block_offset);
   if (bump_step && local_block == 0 && threadIdx.x == 0) P.step[0] += 1;  // Adam step of this iteration
-  for (int i = threadIdx.x; i < P.W; i += blockDim.x) stage_fit_wave(P, i, &s_wave[i]);
-  __syncthreads();
-  adjoint_body<SMAX, KP>(P, local_block, s_wave, s_red);
+  const int ncheb = stage_fit_waves(P, s_wave, s_flag);
+  adjoint_body<SMAX, KP>(P, local_block, s_wave, ncheb, s_red);
   __syncthreads();
   // fixed-order sum over the 4 wavefronts, one partial per (row, block)
   const int rows = P.W * 5 + 1;
@@ -356,25 +424,23 @@ hipError_t launch_zero_words(uint32_t* p, size_t n, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_fit_iteration(const FitLaunch& L, const FitAdam& adam, int mode, float* grad_out, float* loss_out,
-                                hipStream_t st) {
-  const int bump = mode == 1 ? 1 : 0;
-  for (int c = 0; c < kFitClasses; ++c) {
-    if (L.class_blocks[c] == 0) continue;
-    const dim3 grid((unsigned)L.class_blocks[c]);
-    const long long base = L.class_begin[c];
-    if (c == 0)
-      hipLaunchKernelGGL((adjoint_kernel<8, kFitPointsPerThread[0]>), grid, dim3(kFitThreads), 0, st, L.probs,
-                         L.block_prob, base, bump);
-    else if (c == 1)
-      hipLaunchKernelGGL((adjoint_kernel<16, kFitPointsPerThread[1]>), grid, dim3(kFitThreads), 0, st, L.probs,
-                         L.block_prob, base, bump);
-    else
-      hipLaunchKernelGGL((adjoint_kernel<32, kFitPointsPerThread[2]>), grid, dim3(kFitThreads), 0, st, L.probs,
-                         L.block_prob, base, bump);
-    hipError_t e = hipGetLastError();
+// The adjoint launches of every snapshot class present, class C onwards.
+template <int C>
+hipError_t launch_adjoint_classes(const FitLaunch& L, int bump, hipStream_t st) {
+  if (L.class_blocks[C] > 0) {
+    hipLaunchKernelGGL((adjoint_kernel<kFitClassMaxS[C], kFitPointsPerThread[C]>), dim3((unsigned)L.class_blocks[C]),
+                       dim3(kFitThreads), 0, st, L.probs, L.block_prob, L.class_begin[C], bump);
+    const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
+  if constexpr (C + 1 < kFitClasses) return launch_adjoint_classes<C + 1>(L, bump, st);
+  return hipSuccess;
+}
+
+hipError_t launch_fit_iteration(const FitLaunch& L, const FitAdam& adam, int mode, float* grad_out, float* loss_out,
+                                hipStream_t st) {
+  const hipError_t e = launch_adjoint_classes<0>(L, mode == 1 ? 1 : 0, st);
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(reduce_adam_kernel, dim3((unsigned)L.total_rows), dim3(kFitThreads), 0, st, L.probs, L.row_prob,
                      adam, mode, grad_out, loss_out);
   return hipGetLastError();

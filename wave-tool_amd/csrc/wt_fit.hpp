@@ -11,12 +11,14 @@ namespace wt {
 constexpr int kFitMaxWaves = 128;
 constexpr int kFitMaxSnapshots = 32;
 constexpr int kFitThreads = 256;
-// Snapshot classes S <= 8, <= 16, <= 32 and the grid points each thread of their
-// adjoint kernel carries (its residuals, KP x SMAX x 3 floats, stay in registers).
-constexpr int kFitClasses = 3;
-constexpr int kFitClassMaxS[kFitClasses] = {8, 16, 32};
-constexpr int kFitPointsPerThread[kFitClasses] = {2, 1, 1};
-inline int fit_class(int S) { return S <= 8 ? 0 : (S <= 16 ? 1 : 2); }
+// Snapshot classes S <= 4, 8, ..., 32 and the grid points each thread of their adjoint
+// kernel carries (its residuals, KP x SMAX x 3 floats, stay in registers). The kernel
+// of a class steps all SMAX snapshots without branches (residuals past S are zero), so
+// classes 4 snapshots wide waste at most 3 per point.
+constexpr int kFitClasses = 8;
+constexpr int kFitClassMaxS[kFitClasses] = {4, 8, 12, 16, 20, 24, 28, 32};
+constexpr int kFitPointsPerThread[kFitClasses] = {2, 2, 1, 1, 1, 1, 1, 1};
+inline int fit_class(int S) { return (S - 1) / 4; }
 
 // One problem of a batch as the kernels see it. Device pointers are owned by the
 // wt_fit handle except params and target, which the caller owns.

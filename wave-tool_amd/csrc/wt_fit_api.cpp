@@ -62,7 +62,7 @@ wt_status wt_fit_create(wt_ctx* ctx, const wt_fit_problem* problems, uint32_t co
   // state layout per problem: m[W*5], v[W*5], partials[(W*5+1)*nblocks], loss, step
   std::vector<size_t> off(count);
   size_t bytes = 0;
-  long long class_blocks[wt::kFitClasses] = {0, 0, 0};
+  long long class_blocks[wt::kFitClasses] = {};
   for (uint32_t i = 0; i < count; ++i) {
     std::string why;
     if (!problem_ok(problems[i], &why)) {
@@ -101,9 +101,10 @@ wt_status wt_fit_create(wt_ctx* ctx, const wt_fit_problem* problems, uint32_t co
   }
   // blocks grouped by snapshot class: class c occupies [class_begin[c], + class_blocks[c])
   long long class_begin[wt::kFitClasses];
-  for (int c = 0, acc = 0; c < wt::kFitClasses; ++c) {
+  long long acc = 0;
+  for (int c = 0; c < wt::kFitClasses; ++c) {
     class_begin[c] = acc;
-    acc += (int)class_blocks[c];
+    acc += class_blocks[c];
     f->launch.class_begin[c] = class_begin[c];
     f->launch.class_blocks[c] = class_blocks[c];
   }
