@@ -159,7 +159,7 @@ def bench_batched_forward(torch, wt, scenes, frames=120):
     pos = torch.empty(V * 3, device=dev)
     nrm = torch.empty_like(pos)
     # each frame re-aims every patch from its camera on the device (the reference runs its
-    # projector every frame), then one persistent launch displaces all patches
+    # projector every frame), then one launch per segment shape displaces all patches
     cams = torch.from_numpy(wt.camera_rows([s["camera"] for s in specs])).to(dev)
     amps = torch.full((len(specs),), 2.0, dtype=torch.float32, device=dev)
     stream = torch.cuda.Stream()
@@ -199,7 +199,7 @@ def bench_batched_forward(torch, wt, scenes, frames=120):
     batched = vw_frame * frames / wall
     return dict(workload="config2: 512 patches (side 32-1024 log-uniform, 4-128 waves, own camera and t in "
                          "[0, 3600) s), %d frames in one HIP graph; per frame the device projector re-aims "
-                         "every patch, then one persistent launch displaces them" % frames,
+                         "every patch, then one launch per segment shape displaces them" % frames,
                 value=batched, unit="vertex-waves/s", vertex_waves_per_frame=vw_frame,
                 vertices=V, ms_per_frame=wall * 1e3 / frames,
                 device_projector=dict(us_per_frame=jwall * 1e6 / frames, cameras=len(specs),

@@ -264,7 +264,7 @@ typedef struct wt_patch {
 } wt_patch;
 
 /* Validates the patches, uploads them, and builds the work list (units of one
- * workgroup's vertices of one patch, most expensive first) for a persistent launch. */
+ * workgroup's vertices of one patch, grouped by segment shape, most expensive first). */
 wt_status wt_batch_create(wt_ctx* ctx, const wt_patch* patches, uint32_t count, wt_batch** out);
 void wt_batch_destroy(wt_batch* batch);
 /* Total vertices of the batch (the size the output arrays need, in float3). */
@@ -284,9 +284,9 @@ wt_status wt_batch_patch_launch_config(const wt_batch* batch, uint32_t patch, wt
  * captured frame loop moves cameras without a host round trip. */
 wt_status wt_batch_set_cameras(wt_ctx* ctx, wt_batch* batch, const wt_camera* d_cameras, const float* d_amplitudes,
                                int32_t* d_status, void* stream);
-/* One persistent launch over every patch: each patch's vertex program at
- * time + time_offset. d_waves holds total_waves waves that the patches index into.
- * Replayable from a captured graph (the work counter resets itself). */
+/* Every patch's vertex program at time + time_offset, in one launch per segment
+ * shape (one workgroup per unit, dispatched largest first). d_waves holds total_waves waves that the patches index into.
+ * Replayable from a captured graph (no device state changes between calls). */
 wt_status wt_water_displace_batched(wt_ctx* ctx, wt_batch* batch, const wt_gerstner_wave* d_waves, uint32_t total_waves,
                                     const wt_heightmap* heightmap, const wt_vertex_buffers* out, double time_offset,
                                     void* stream);

@@ -1,4 +1,4 @@
-"""GPU tests of the persistent batched forward kernel (wt_water_displace_batched)."""
+"""GPU tests of the batched forward kernels (wt_water_displace_batched)."""
 import os
 
 import numpy as np
@@ -94,8 +94,7 @@ def test_batched_oracle_parity_long_times(torch_cuda, wtctx):
 
 
 def test_batched_graph_replay(torch_cuda, wtctx):
-    """Eight frames of the batch captured in one graph; replays equal eager frames (the
-    work counter resets itself at the end of every launch)."""
+    """Eight frames of the batch captured in one graph; replays equal eager frames."""
     torch = torch_cuda
     say("batched graph replay")
     specs = scenes.batched_patches(seed=11, count=12)

@@ -482,8 +482,8 @@ struct wt_batch {
   std::vector<wt::PatchDesc> patches;
   wt::PatchDesc* d_patches = nullptr;
   wt::BatchUnit* d_units = nullptr;
-  unsigned* d_counters = nullptr;
-  int n_units = 0;
+  long long shape_begin[wt::kBatchShapes] = {};
+  int shape_units[wt::kBatchShapes] = {};
   uint64_t vertices = 0;
   uint32_t max_wave_end = 0;
 };
@@ -542,22 +542,26 @@ wt_status wt_batch_create(wt_ctx* ctx, const wt_patch* patches, uint32_t count, 
     const long long groups = wt::forward_segment_groups(d.N, s);
     for (long long gidx = 0; gidx < groups; ++gidx) units.push_back(wt::BatchUnit{(int)i, (int)gidx});
   }
-  // most expensive units first: a unit costs about wave count x vertices per segment
-  auto cost = [&](const wt::BatchUnit& x) {
-    const wt::PatchDesc& d = b->patches[x.patch];
-    const wt::LaunchShape& s = wt::kBatchShape[d.shape];
-    return (double)d.W * s.lanes_per_row * s.steps * (64 / s.lanes_per_row);
-  };
-  std::stable_sort(units.begin(), units.end(),
-                   [&](const wt::BatchUnit& x, const wt::BatchUnit& y) { return cost(x) > cost(y); });
-  b->n_units = (int)units.size();
+  // grouped by shape (one launch each), most expensive units first within a shape: a
+  // unit costs about its patch's wave count
+  auto shape_of = [&](const wt::BatchUnit& x) { return b->patches[x.patch].shape; };
+  std::stable_sort(units.begin(), units.end(), [&](const wt::BatchUnit& x, const wt::BatchUnit& y) {
+    if (shape_of(x) != shape_of(y)) return shape_of(x) < shape_of(y);
+    return b->patches[x.patch].W > b->patches[y.patch].W;
+  });
+  if (units.size() >= (1ull << 31)) {
+    delete b;
+    return fail(ctx, WT_ERR_UNSUPPORTED, "batch too large: more than 2^31 workgroup units");
+  }
+  for (size_t k = 0; k < units.size(); ++k) {
+    const int sh = shape_of(units[k]);
+    if (b->shape_units[sh]++ == 0) b->shape_begin[sh] = (long long)k;
+  }
   hipError_t e = hipMalloc(&b->d_patches, sizeof(wt::PatchDesc) * count);
   if (e == hipSuccess) e = hipMalloc(&b->d_units, sizeof(wt::BatchUnit) * units.size());
-  if (e == hipSuccess) e = hipMalloc(&b->d_counters, 2 * sizeof(unsigned));
   if (e == hipSuccess)
     e = hipMemcpy(b->d_patches, b->patches.data(), sizeof(wt::PatchDesc) * count, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(b->d_units, units.data(), sizeof(wt::BatchUnit) * units.size(), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemset(b->d_counters, 0, 2 * sizeof(unsigned));
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) {
     wt_status s = hip_fail(ctx, e, "wt_batch_create");
@@ -572,7 +576,6 @@ void wt_batch_destroy(wt_batch* b) {
   if (!b) return;
   if (b->d_patches) (void)hipFree(b->d_patches);
   if (b->d_units) (void)hipFree(b->d_units);
-  if (b->d_counters) (void)hipFree(b->d_counters);
   delete b;
 }
 
@@ -682,8 +685,6 @@ wt_status wt_water_displace_batched(wt_ctx* ctx, wt_batch* b, const wt_gerstner_
   wt::BatchArgs a{};
   a.patches = b->d_patches;
   a.units = b->d_units;
-  a.n_units = b->n_units;
-  a.counters = b->d_counters;
   a.time_offset = time_offset;
   a.waves = d_waves;
   if (hm) {
@@ -697,12 +698,13 @@ wt_status wt_water_displace_batched(wt_ctx* ctx, wt_batch* b, const wt_gerstner_
   a.nrm = out->normal;
   a.tan = out->tangent;
   a.bit = out->bitangent;
-  int cus = 0;
-  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || cus <= 0) cus = 256;
-  const long long resident = (long long)wt::batched_blocks_per_cu(wt::tex_mode(a.texels, a.tex_fmt, a.tex_wrap)) * cus;
-  const int grid = (int)std::min<long long>(resident, b->n_units);
-  hipError_t e = wt::launch_forward_batched(a, grid, st);
-  if (e != hipSuccess) return hip_fail(ctx, e, "water_forward_batched_kernel launch");
+  for (int sh = 0; sh < wt::kBatchShapes; ++sh) {
+    if (b->shape_units[sh] == 0) continue;
+    a.unit_begin = b->shape_begin[sh];
+    a.n_units = b->shape_units[sh];
+    hipError_t e = wt::launch_forward_batched(a, sh, st);
+    if (e != hipSuccess) return hip_fail(ctx, e, "water_forward_batched_kernel launch");
+  }
   return WT_OK;
 }
 

@@ -57,6 +57,18 @@ __device__ __forceinline__ void stage_wave(const wt_gerstner_wave& g, double t, 
   o->cs[3] = qaw * Dz * Dz;
 }
 
+// Entry i of a table that no thread writes during the launch, read through the constant
+// address space: a wave-uniform index then loads it with scalar loads into SGPRs.
+template <class T>
+__device__ __forceinline__ T load_uniform(const T* table, long long i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef const __attribute__((address_space(4))) T ConstT;
+  return ((ConstT*)table)[i];
+#else
+  return table[i];  // the host pass only parses device code
+#endif
+}
+
 // Heightmap sampling mode, a compile-time parameter of the kernels: none, or texel
 // format x wrap mode (src/texture.cpp:54-72 sets the reference's to 8-bit, MIRRORED_REPEAT).
 enum TexMode : int { kTexNone = 0, kTexR32Repeat = 1, kTexR32Mirror = 2, kTexR8Repeat = 3, kTexR8Mirror = 4 };

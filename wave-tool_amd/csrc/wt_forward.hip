@@ -1,6 +1,5 @@
-// Launch dispatch of the forward kernels (templates in wt_forward.hpp): the plain
-// kernel per texture mode (one translation unit each, wt_forward_t*.hip) and the
-// persistent batched kernel.
+// Launch dispatch of the forward kernels (templates in wt_forward.hpp): the plain and
+// batched kernels per texture mode (one translation unit each, wt_forward_t*.hip).
 #include "wt_forward.hpp"
 
 namespace wt {
@@ -49,35 +48,20 @@ int batch_shape_for(int N) {
   return pick;
 }
 
-hipError_t launch_forward_batched(const BatchArgs& a, int grid, hipStream_t st) {
-  const dim3 g(grid), b(256);
+hipError_t launch_forward_batched_t0(const BatchArgs& a, int shape, hipStream_t st);
+hipError_t launch_forward_batched_t1(const BatchArgs& a, int shape, hipStream_t st);
+hipError_t launch_forward_batched_t2(const BatchArgs& a, int shape, hipStream_t st);
+hipError_t launch_forward_batched_t3(const BatchArgs& a, int shape, hipStream_t st);
+hipError_t launch_forward_batched_t4(const BatchArgs& a, int shape, hipStream_t st);
+
+hipError_t launch_forward_batched(const BatchArgs& a, int shape, hipStream_t st) {
+  if (a.n_units <= 0) return hipSuccess;
   switch (tex_mode(a.texels, a.tex_fmt, a.tex_wrap)) {
-    case kTexR32Repeat: hipLaunchKernelGGL((water_forward_batched_kernel<kTexR32Repeat>), g, b, 0, st, a); break;
-    case kTexR32Mirror: hipLaunchKernelGGL((water_forward_batched_kernel<kTexR32Mirror>), g, b, 0, st, a); break;
-    case kTexR8Repeat: hipLaunchKernelGGL((water_forward_batched_kernel<kTexR8Repeat>), g, b, 0, st, a); break;
-    case kTexR8Mirror: hipLaunchKernelGGL((water_forward_batched_kernel<kTexR8Mirror>), g, b, 0, st, a); break;
-    default: hipLaunchKernelGGL((water_forward_batched_kernel<kTexNone>), g, b, 0, st, a); break;
-  }
-  return hipGetLastError();
-}
-
-template <int kTex>
-static int occupancy_kernel() {
-  int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, water_forward_batched_kernel<kTex>, 256, 0) != hipSuccess) {
-    (void)hipGetLastError();
-    return 1;
-  }
-  return n > 0 ? n : 1;
-}
-
-int batched_blocks_per_cu(int tex) {
-  switch (tex) {
-    case kTexR32Repeat: return occupancy_kernel<kTexR32Repeat>();
-    case kTexR32Mirror: return occupancy_kernel<kTexR32Mirror>();
-    case kTexR8Repeat: return occupancy_kernel<kTexR8Repeat>();
-    case kTexR8Mirror: return occupancy_kernel<kTexR8Mirror>();
-    default: return occupancy_kernel<kTexNone>();
+    case kTexR32Repeat: return launch_forward_batched_t1(a, shape, st);
+    case kTexR32Mirror: return launch_forward_batched_t2(a, shape, st);
+    case kTexR8Repeat: return launch_forward_batched_t3(a, shape, st);
+    case kTexR8Mirror: return launch_forward_batched_t4(a, shape, st);
+    default: return launch_forward_batched_t0(a, shape, st);
   }
 }
 

@@ -351,6 +351,14 @@ __global__ __launch_bounds__(256) void water_forward_kernel(FwdParams p) {
 
 // ------------------------------------------------------------------ batched
 // Forward parameters of one patch of a batch (corner algebra done on the host).
+// A wave-uniform double held in VGPRs, moved to SGPRs.
+__device__ __forceinline__ double uniform_f64(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 __device__ __forceinline__ FwdParams patch_params(const BatchArgs& a, const PatchDesc& pd) {
   FwdParams p;
   p.bl_x = pd.bl_x; p.bl_z = pd.bl_z;
@@ -358,7 +366,7 @@ __device__ __forceinline__ FwdParams patch_params(const BatchArgs& a, const Patc
   p.dv_x = pd.dv_x; p.dv_z = pd.dv_z;
   p.duv_x = pd.duv_x; p.duv_z = pd.duv_z;
   p.inv_nm1 = pd.inv_nm1;
-  p.time = pd.time + a.time_offset;
+  p.time = uniform_f64(pd.time + a.time_offset);
   p.N = pd.N;
   p.W = pd.W;
   p.waves = a.waves + pd.wave_offset;
@@ -372,8 +380,9 @@ __device__ __forceinline__ FwdParams patch_params(const BatchArgs& a, const Patc
   // texture-space scroll offset -s_h v t reduced to one wrap period in FP64
   const double period = a.tex_wrap == WT_WRAP_REPEAT ? 1.0 : 2.0;
   const double os = -(double)pd.s_h * (double)pd.scroll_x * p.time, ot = (double)pd.s_h * (double)pd.scroll_z * p.time;
-  e.hm_off_s = os - period * floor(os / period);
-  e.hm_off_t = ot - period * floor(ot / period);
+  // uniform FP64 results of the VALU go back to scalar registers
+  e.hm_off_s = uniform_f64(os - period * floor(os / period));
+  e.hm_off_t = uniform_f64(ot - period * floor(ot / period));
   const size_t base = 3 * (size_t)pd.vertex_offset;
   e.pos = a.pos + base;
   e.nrm = a.nrm + base;
@@ -382,48 +391,37 @@ __device__ __forceinline__ FwdParams patch_params(const BatchArgs& a, const Patc
   return p;
 }
 
-// Persistent kernel over a batch of patches: each workgroup takes the next unit (one
-// workgroup's segments of one patch; units are sorted by cost, largest first) from a
-// device counter until the list is exhausted, and runs it with its patch's segment shape
-// (kBatchShape). The last workgroup to finish resets the counters, so the launch can be
-// replayed from a captured graph without a memset node.
-constexpr size_t kBatchLds = cmax(cmax(sizeof(SegShared<8, 64, 2>), sizeof(SegShared<4, 32, 2>)),
-                                  sizeof(SegShared<4, 16, 2>));
-
-template <int kTex>
+// Batched kernel over the patches of a batch that use segment shape kShape
+// (kBatchShape): workgroup b runs unit b (one workgroup's segments of one patch). The
+// units are sorted by cost, largest first, so the hardware dispatcher, which starts
+// workgroups in index order as slots free up, balances them like a longest-first work
+// queue, with no atomics: a device-wide unit counter serialises every fetch through one
+// address and measured twice as slow as this on a single config-1-sized patch. One
+// kernel per shape keeps each at its own register count and LDS size.
+template <int kTex, int kShape>
 __global__ __launch_bounds__(256) void water_forward_batched_kernel(BatchArgs a) {
-  __shared__ __attribute__((aligned(16))) unsigned char raw[kBatchLds];
-  __shared__ int s_unit;
-  for (;;) {
-    if (threadIdx.x == 0) s_unit = (int)atomicAdd(&a.counters[0], 1u);
-    __syncthreads();
-    const int u = __builtin_amdgcn_readfirstlane(s_unit);  // wave-uniform: patch fields load as scalars
-    __syncthreads();  // every thread has read s_unit before it is overwritten
-    if (u >= a.n_units) break;
-    const BatchUnit unit = a.units[u];
-    const PatchDesc& pd = a.patches[unit.patch];
-    if (!pd.active) continue;  // workgroup-uniform: every thread takes the next unit
-    const FwdParams p = patch_params(a, pd);
-    switch (pd.shape) {
-      case 0:
-        forward_segments<8, 64, 2, kTex>(p, (long long)unit.group, *reinterpret_cast<SegShared<8, 64, 2>*>(raw));
-        break;
-      case 1:
-        forward_segments<4, 32, 2, kTex>(p, (long long)unit.group, *reinterpret_cast<SegShared<4, 32, 2>*>(raw));
-        break;
-      default:
-        forward_segments<4, 16, 2, kTex>(p, (long long)unit.group, *reinterpret_cast<SegShared<4, 16, 2>*>(raw));
-        break;
-    }
+  constexpr int K = kBatchShape[kShape].steps, LC = kBatchShape[kShape].lanes_per_row;
+  static_assert(kBatchShape[kShape].split == 2, "batched shapes pair two wavefronts per segment");
+  __shared__ SegShared<K, LC, 2> lds;
+  // the unit and patch tables do not change during the launch: read them through the
+  // constant address space so they load with scalar loads into SGPRs (a vector load
+  // would park the whole patch descriptor in VGPRs)
+  const BatchUnit unit = load_uniform(a.units, a.unit_begin + (long long)blockIdx.x);
+  const PatchDesc pd = load_uniform(a.patches, unit.patch);
+  if (!pd.active) return;  // workgroup-uniform
+  forward_segments<K, LC, 2, kTex>(patch_params(a, pd), (long long)unit.group, lds);
+}
+
+// The batched kernel of one texture mode in segment shape `shape`, one workgroup per unit.
+template <int kTex>
+hipError_t launch_batched_mode(const BatchArgs& a, int shape, hipStream_t st) {
+  const dim3 g((unsigned)a.n_units), b(256);
+  switch (shape) {
+    case 0: hipLaunchKernelGGL((water_forward_batched_kernel<kTex, 0>), g, b, 0, st, a); break;
+    case 1: hipLaunchKernelGGL((water_forward_batched_kernel<kTex, 1>), g, b, 0, st, a); break;
+    default: hipLaunchKernelGGL((water_forward_batched_kernel<kTex, 2>), g, b, 0, st, a); break;
   }
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const unsigned done = atomicAdd(&a.counters[1], 1u);
-    if (done == gridDim.x - 1) {
-      atomicExch(&a.counters[0], 0u);
-      atomicExch(&a.counters[1], 0u);
-    }
-  }
+  return hipGetLastError();
 }
 
 // ------------------------------------------------------------------ launch helpers

@@ -66,9 +66,9 @@ struct BatchUnit {
 
 struct BatchArgs {
   const PatchDesc* patches;
-  const BatchUnit* units;
+  const BatchUnit* units;  // every unit of the batch, grouped by segment shape
+  long long unit_begin;    // this launch's units: [unit_begin, unit_begin + n_units)
   int n_units;
-  unsigned* counters;  // [0] next unit, [1] workgroups finished
   double time_offset;  // added to every patch's time (frame clock)
   const wt_gerstner_wave* waves;
   const void* texels;
@@ -127,9 +127,8 @@ hipError_t launch_projector(const ProjectorArgs& a, hipStream_t st);
 // Triangle index list of an n x n grid (n >= 2), 6 (n - 1)^2 entries, ids offset by base.
 hipError_t launch_grid_indices(uint32_t n, uint32_t base, uint32_t* out, hipStream_t st);
 
-// Launches the persistent batched kernel with `grid` workgroups.
-hipError_t launch_forward_batched(const BatchArgs& a, int grid, hipStream_t st);
-// Workgroups of the batched kernel resident per CU.
-int batched_blocks_per_cu(int tex_mode);
+// Launches the batched kernel of segment shape `shape` (kBatchShape), one workgroup per
+// unit of a.
+hipError_t launch_forward_batched(const BatchArgs& a, int shape, hipStream_t st);
 
 }  // namespace wt

@@ -242,7 +242,7 @@ def batch_inputs(specs, amp=2.0, sample_scale=0.02, displacement_scale=0.05, scr
 
 
 class Batch:
-    """A batch of patches for the persistent batched forward kernel (wt_batch)."""
+    """A batch of patches for the batched forward kernels (wt_batch)."""
 
     def __init__(self, ctx, patches):
         self.ctx = ctx
