@@ -207,7 +207,8 @@ typedef struct wt_varying_buffers {
   float* heightmap_sample;     /* float: the heightmap value sampled at P (heightmap_colour.r) */
 } wt_varying_buffers;
 
-/* One HBM-bound pass over vertex_count vertices (positions / normals as float3 arrays).
+/* One HBM-bound pass over vertex_count (at most 2^34) vertices (positions / normals as
+ * float3 arrays).
  * view_projection = P V of the camera and view_only_yaw = V of the same camera with
  * pitch 0 (src/render-engine.cpp:278-284), both 4x4 column-major floats as the
  * reference uploads them (wt_camera_matrices gives V and P in double). params supplies

@@ -297,7 +297,9 @@ __global__ __launch_bounds__(kFitThreads) void adjoint_kernel(const FitDesc* __r
   __shared__ float s_red[(kFitMaxWaves * 5 + 1) * 4];
   __shared__ int s_flag[kFitMaxWaves];
   const long long gb = base + blockIdx.x;
-  const FitDesc P = probs[block_prob[gb]];
+  // the problem and block tables do not change during the launch: scalar loads, so the
+  // descriptor sits in SGPRs rather than VGPRs
+  const FitDesc P = load_uniform(probs, load_uniform(block_prob, gb));
   const int local_block = (int)(gb - P.block_offset);
   if (bump_step && local_block == 0 && threadIdx.x == 0) P.step[0] += 1;  // Adam step of this iteration
   const int ncheb = stage_fit_waves(P, s_wave, s_flag);
@@ -316,8 +318,8 @@ __global__ __launch_bounds__(kFitThreads) void reduce_adam_kernel(const FitDesc*
                                                                   int mode, float* __restrict__ grad_out,
                                                                   float* __restrict__ loss_out) {
   __shared__ float s[kFitThreads];
-  const int s_prob = row_prob[blockIdx.x];
-  const FitDesc P = probs[s_prob];
+  const int s_prob = load_uniform(row_prob, (long long)blockIdx.x);
+  const FitDesc P = load_uniform(probs, s_prob);
   const int row = (int)(blockIdx.x - P.row_offset);
   const float* src = P.partials + (size_t)row * P.nblocks;
   float acc = 0.0f;

@@ -254,7 +254,7 @@ wt_status wt_vertex_varyings(wt_ctx* ctx, const wt_water_params* p, const wt_hei
                              const wt_varying_buffers* out, void* stream) {
   if (!ctx) return WT_ERR_INVALID_ARG;
   if (!p || !out || !vp || !vyaw) return fail(ctx, WT_ERR_INVALID_ARG, "params, matrices and out must be non-NULL");
-  if (count > (1ull << 40)) return fail(ctx, WT_ERR_INVALID_ARG, "vertex_count too large");
+  if (count > (1ull << 34)) return fail(ctx, WT_ERR_INVALID_ARG, "vertex_count too large (limit 2^34)");
   for (int i = 0; i < 16; ++i)
     if (!std::isfinite(vp[i]) || !std::isfinite(vyaw[i])) return fail(ctx, WT_ERR_INVALID_ARG, "matrices must be finite");
   const float fs[6] = {p->cameraPosition[0], p->cameraPosition[1], p->cameraPosition[2], p->heightmapSampleScale,
