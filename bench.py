@@ -38,9 +38,9 @@ def cpu_threads():
     return min(16, os.cpu_count() or 1)
 
 
-def cpu_baseline(corners, cam_pos, waves, texels, cfg, max_frames=16, budget_s=2.5):
+def cpu_baseline(corners, cam_pos, waves, texels, cfg, max_frames=240, budget_s=10.0):
     """FP32 oracle instantiation, all host threads (up to 16), frames of the same
-    workload until the time budget or max_frames is reached."""
+    workload (t = 10 + k/60) until about 10 s of wall time or the step's 240 frames."""
     import oracle as orc
     threads = cpu_threads()
     N, W = cfg["N"], cfg["W"]
