@@ -153,7 +153,11 @@ def bench_batched_forward(torch, wt, scenes, frames=120):
     patches, waves_np, V = wt.batch_inputs(specs)
     vw_frame = sum(s["N"] * s["N"] * s["W"] for s in specs)
     ctx = wt.Context()
+    # segment shapes from the padded-vertex model: on this batch it beats the per-bucket
+    # autotuner (wt_autotune_batch), whose synthetic same-size batches favour the K = 4
+    # shapes more than the mixed batch does
     batch = wt.Batch(ctx, patches)
+    shapes = [batch.patch_launch_config(i) for i in range(len(patches))]
     waves = torch.from_numpy(waves_np).to(dev)
     hm = torch.from_numpy(scenes.value_noise(1, 512)).to(dev)
     pos = torch.empty(V * 3, device=dev)
@@ -202,6 +206,7 @@ def bench_batched_forward(torch, wt, scenes, frames=120):
                          "every patch, then one launch per segment shape displaces them" % frames,
                 value=batched, unit="vertex-waves/s", vertex_waves_per_frame=vw_frame,
                 vertices=V, ms_per_frame=wall * 1e3 / frames,
+                patches_per_shape={"%dx%d" % (sh[2] * sh[1], 64 // sh[2]): shapes.count(sh) for sh in set(shapes)},
                 device_projector=dict(us_per_frame=jwall * 1e6 / frames, cameras=len(specs),
                                       note="wt_batch_set_cameras alone, %d launches in one HIP graph" % frames),
                 plain_per_patch=dict(value=vw_frame / pwall, unit="vertex-waves/s", ms_per_frame=pwall * 1e3,

@@ -230,7 +230,17 @@ wt_status wt_get_launch_config(wt_ctx* ctx, uint32_t grid_length, uint32_t wave_
  * per-launch time. Allocates and frees its own scratch buffers. */
 wt_status wt_autotune_forward(wt_ctx* ctx, uint32_t grid_length, uint32_t wave_count, int with_heightmap,
                               wt_launch_config* best, float* best_ms);
-/* Save / load the autotune table as text (one line per size bucket). */
+/* The batched kernels' segment shape for patches of side ~grid_length with
+ * ~wave_count waves: times each of the WT_BATCH_SHAPES shapes on a synthetic batch of
+ * such patches (about 4 M vertices) and records the fastest for that (N, W) size
+ * bucket (powers of two); wt_batch_create then uses it for patches in the bucket,
+ * and a padded-vertex model for buckets never tuned. best_shape receives the index
+ * (the shape itself through wt_batch_patch_launch_config); no reference counterpart. */
+#define WT_BATCH_SHAPES 3
+wt_status wt_autotune_batch(wt_ctx* ctx, uint32_t grid_length, uint32_t wave_count, int with_heightmap,
+                            uint32_t* best_shape, float* best_ms);
+/* Save / load the autotune tables (plain and batched) as text, one line per size
+ * bucket. */
 wt_status wt_autotune_save(wt_ctx* ctx, const char* path);
 wt_status wt_autotune_load(wt_ctx* ctx, const char* path);
 

@@ -23,6 +23,7 @@ struct wt_ctx {
   bool forced = false;
   wt::LaunchShape forced_shape{};
   std::map<std::pair<int, int>, wt::LaunchShape> table;  // (N bucket, W bucket) -> shape
+  std::map<std::pair<int, int>, int> batch_table;        // (N bucket, W bucket) -> batch shape
 };
 
 struct wt_graph {
@@ -449,6 +450,8 @@ wt_status wt_autotune_save(wt_ctx* ctx, const char* path) {
   for (const auto& kv : ctx->table)
     std::fprintf(f, "%d %d %d %d %d %d\n", kv.first.first, kv.first.second, kv.second.split, kv.second.steps,
                  kv.second.lanes_per_row, kv.second.wave_chunk);
+  std::fprintf(f, "# batched: b n_bucket w_bucket batch_shape\n");
+  for (const auto& kv : ctx->batch_table) std::fprintf(f, "b %d %d %d\n", kv.first.first, kv.first.second, kv.second);
   std::fclose(f);
   return WT_OK;
 }
@@ -459,9 +462,19 @@ wt_status wt_autotune_load(wt_ctx* ctx, const char* path) {
   if (!f) return fail(ctx, WT_ERR_INVALID_ARG, std::string("cannot open ") + path);
   char line[256];
   std::map<std::pair<int, int>, wt::LaunchShape> t;
+  std::map<std::pair<int, int>, int> bt;
   while (std::fgets(line, sizeof line, f)) {
     if (line[0] == '#') continue;
     int nb, wb;
+    if (line[0] == 'b') {
+      int shape;
+      if (std::sscanf(line + 1, "%d %d %d", &nb, &wb, &shape) != 3 || shape < 0 || shape >= wt::kBatchShapes) {
+        std::fclose(f);
+        return fail(ctx, WT_ERR_INVALID_ARG, "malformed autotune table");
+      }
+      bt[{nb, wb}] = shape;
+      continue;
+    }
     wt::LaunchShape s{};
     if (std::sscanf(line, "%d %d %d %d %d %d", &nb, &wb, &s.split, &s.steps, &s.lanes_per_row, &s.wave_chunk) != 6 ||
         !valid_shape(s)) {
@@ -472,6 +485,7 @@ wt_status wt_autotune_load(wt_ctx* ctx, const char* path) {
   }
   std::fclose(f);
   ctx->table = t;
+  ctx->batch_table = bt;
   return WT_OK;
 }
 
@@ -488,9 +502,10 @@ struct wt_batch {
   uint32_t max_wave_end = 0;
 };
 
-extern "C" {
-
-wt_status wt_batch_create(wt_ctx* ctx, const wt_patch* patches, uint32_t count, wt_batch** out) {
+// forced_shape < 0: each patch's shape from the context's batch autotune table, else
+// the padded-vertex model (batch_shape_for)
+static wt_status batch_create(wt_ctx* ctx, const wt_patch* patches, uint32_t count, int forced_shape,
+                              wt_batch** out) {
   if (!ctx || !out) return WT_ERR_INVALID_ARG;
   *out = nullptr;
   if (!patches || count == 0) return fail(ctx, WT_ERR_INVALID_ARG, "need at least one patch");
@@ -528,7 +543,12 @@ wt_status wt_batch_create(wt_ctx* ctx, const wt_patch* patches, uint32_t count, 
     d.W = (int)p.wave_count;
     d.wave_offset = (int)p.wave_offset;
     d.vertex_offset = (long long)p.vertex_offset;
-    d.shape = wt::batch_shape_for(d.N);
+    if (forced_shape >= 0) {
+      d.shape = forced_shape;
+    } else {
+      auto it = ctx->batch_table.find({bucket_log2(p.gridLength), bucket_log2(p.wave_count ? p.wave_count : 1)});
+      d.shape = it != ctx->batch_table.end() ? it->second : wt::batch_shape_for(d.N);
+    }
     b->patches.push_back(d);
     b->vertices = std::max<uint64_t>(b->vertices, p.vertex_offset + (uint64_t)p.gridLength * p.gridLength);
     b->max_wave_end = std::max<uint32_t>(b->max_wave_end, p.wave_offset + p.wave_count);
@@ -572,6 +592,12 @@ wt_status wt_batch_create(wt_ctx* ctx, const wt_patch* patches, uint32_t count, 
   return WT_OK;
 }
 
+extern "C" {
+
+wt_status wt_batch_create(wt_ctx* ctx, const wt_patch* patches, uint32_t count, wt_batch** out) {
+  return batch_create(ctx, patches, count, -1, out);
+}
+
 void wt_batch_destroy(wt_batch* b) {
   if (!b) return;
   if (b->d_patches) (void)hipFree(b->d_patches);
@@ -592,6 +618,111 @@ wt_status wt_batch_patch_launch_config(const wt_batch* b, uint32_t patch, wt_lau
   cfg->steps_per_lane = (uint32_t)s.steps;
   cfg->lanes_per_row = (uint32_t)s.lanes_per_row;
   cfg->wave_chunk = (uint32_t)s.wave_chunk;
+  return WT_OK;
+}
+
+wt_status wt_autotune_batch(wt_ctx* ctx, uint32_t N, uint32_t W, int with_hm, uint32_t* best_shape, float* best_ms) {
+  if (!ctx) return WT_ERR_INVALID_ARG;
+  if (N < 2 || N > 8192 || W > WT_MAX_WAVES) return fail(ctx, WT_ERR_INVALID_ARG, "autotune size out of range");
+  // enough patches of side N for about 4 M vertices: every shape then has thousands of
+  // workgroup units, as in a real batch
+  const uint64_t V1 = (uint64_t)N * N;
+  const uint32_t P = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(4096, (4ull << 20) / V1));
+  const uint64_t V = V1 * P;
+  float *pos = nullptr, *nrm = nullptr, *tex = nullptr;
+  wt_gerstner_wave* dw = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  wt_batch* b = nullptr;
+  const uint32_t TW = 256;
+  std::vector<wt_gerstner_wave> hw(W ? W : 1);
+  for (uint32_t i = 0; i < W; ++i) {  // the forward autotuner's synthetic wave set
+    const float L = 1.0f + 59.0f * (float)((i * 37u) % 101u) / 100.0f;
+    const float ang = 0.7f * std::sin(1.7f * (float)i);
+    hw[i] = wt_gerstner_wave{L / 90.0f, 6.2831853f / L, std::sqrt(9.81f * 6.2831853f / L), 0.5f / (float)W,
+                             {std::cos(ang), std::sin(ang)}, 0.3f * (float)i, 0.0f};
+  }
+  std::vector<float> htex(TW * TW);
+  for (uint32_t i = 0; i < TW * TW; ++i) htex[i] = 0.5f + 0.5f * std::sin(0.01f * (float)i);
+  std::vector<wt_patch> patches(P);
+  const double cc[4][4] = {{-60, 0, -10, 1}, {-900, 0, -1000, 1}, {60, 0, -10, 1}, {900, 0, -1000, 1}};
+  for (uint32_t i = 0; i < P; ++i) {
+    wt_patch& q = patches[i];
+    q = wt_patch{};
+    std::memcpy(q.corners, cc, sizeof(cc));
+    q.cameraPosition[1] = 10.0f;
+    q.gridLength = N;
+    q.wave_offset = 0;
+    q.wave_count = W;
+    q.vertex_offset = (uint64_t)i * V1;
+    q.waveAnimationTimeInSeconds = 10.0 + 0.37 * i;
+    q.heightmapSampleScale = 0.02f;
+    q.heightmapDisplacementScale = 0.05f;
+  }
+  wt_status rs = WT_OK;
+  auto cleanup = [&]() {
+    if (b) wt_batch_destroy(b);
+    if (pos) (void)hipFree(pos);
+    if (nrm) (void)hipFree(nrm);
+    if (tex) (void)hipFree(tex);
+    if (dw) (void)hipFree(dw);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+  };
+#define WT_TRY(x)                 \
+  do {                            \
+    hipError_t _e = (x);          \
+    if (_e != hipSuccess) {       \
+      rs = hip_fail(ctx, _e, #x); \
+      cleanup();                  \
+      return rs;                  \
+    }                             \
+  } while (0)
+  WT_TRY(hipMalloc(&pos, V * 12));
+  WT_TRY(hipMalloc(&nrm, V * 12));
+  WT_TRY(hipMalloc(&dw, sizeof(wt_gerstner_wave) * hw.size()));
+  WT_TRY(hipMalloc(&tex, sizeof(float) * htex.size()));
+  WT_TRY(hipMemcpy(dw, hw.data(), sizeof(wt_gerstner_wave) * hw.size(), hipMemcpyHostToDevice));
+  WT_TRY(hipMemcpy(tex, htex.data(), sizeof(float) * htex.size(), hipMemcpyHostToDevice));
+  WT_TRY(hipStreamCreate(&st));
+  WT_TRY(hipEventCreate(&e0));
+  WT_TRY(hipEventCreate(&e1));
+  const wt_heightmap hm{tex, TW, TW, WT_TEXEL_R32F, WT_WRAP_REPEAT};
+  const wt_vertex_buffers vb{pos, nrm, nullptr, nullptr};
+  float best_t = 1e30f;
+  int best_s = wt::batch_shape_for((int)N);
+  for (int sh = 0; sh < wt::kBatchShapes; ++sh) {
+    rs = batch_create(ctx, patches.data(), P, sh, &b);
+    if (rs != WT_OK) {
+      cleanup();
+      return rs;
+    }
+    rs = wt_water_displace_batched(ctx, b, dw, W, with_hm ? &hm : nullptr, &vb, 0.0, st);  // warm-up
+    if (rs != WT_OK) {
+      cleanup();
+      return rs;
+    }
+    const int reps = 5;
+    WT_TRY(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; ++r) (void)wt_water_displace_batched(ctx, b, dw, W, with_hm ? &hm : nullptr, &vb, r / 60.0, st);
+    WT_TRY(hipEventRecord(e1, st));
+    WT_TRY(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    WT_TRY(hipEventElapsedTime(&ms, e0, e1));
+    ms /= reps;
+    if (ms < best_t) {
+      best_t = ms;
+      best_s = sh;
+    }
+    wt_batch_destroy(b);
+    b = nullptr;
+  }
+#undef WT_TRY
+  ctx->batch_table[{bucket_log2(N), bucket_log2(W ? W : 1)}] = best_s;
+  if (best_shape) *best_shape = (uint32_t)best_s;
+  if (best_ms) *best_ms = best_t;
+  cleanup();
   return WT_OK;
 }
 

@@ -104,7 +104,7 @@ long long forward_segment_groups(int N, const LaunchShape& s);
 
 // Segment shapes the batched kernel chooses from, per patch (all with S = 2): 512 x 1
 // (K = 8) for wide grids, 128 x 2 and 64 x 4 (K = 4) for narrower ones.
-constexpr int kBatchShapes = 3;
+constexpr int kBatchShapes = WT_BATCH_SHAPES;
 constexpr LaunchShape kBatchShape[kBatchShapes] = {{2, 8, 64, 64}, {2, 4, 32, 64}, {2, 4, 16, 64}};
 // The shape with the fewest padded vertex-waves for a grid of side N (a small premium
 // on the K = 4 shapes for their extra seeding).

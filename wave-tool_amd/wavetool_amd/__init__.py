@@ -192,6 +192,20 @@ class Context:
         return dict(wave_split=cfg.wave_split, steps=cfg.steps_per_lane, lanes_per_row=cfg.lanes_per_row,
                     wave_chunk=cfg.wave_chunk), ms.value
 
+    def autotune_batch(self, N, W, with_heightmap=True):
+        """Batched segment shape (index) for the (N, W) size bucket, and its time (ms)."""
+        shape = ctypes.c_uint32()
+        ms = ctypes.c_float()
+        self._check(lib().wt_autotune_batch(self.handle, int(N), int(W), int(bool(with_heightmap)),
+                                            ctypes.byref(shape), ctypes.byref(ms)))
+        return shape.value, ms.value
+
+    def autotune_save(self, path):
+        self._check(lib().wt_autotune_save(self.handle, str(path).encode()))
+
+    def autotune_load(self, path):
+        self._check(lib().wt_autotune_load(self.handle, str(path).encode()))
+
     def graph_begin(self, stream):
         self._check(lib().wt_graph_begin(self.handle, ctypes.c_void_p(stream.cuda_stream)))
 

@@ -169,6 +169,9 @@ PROTOTYPES = [
     ("wt_autotune_forward", ctypes.c_int,
      [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(LaunchConfig),
       ctypes.POINTER(ctypes.c_float)]),
+    ("wt_autotune_batch", ctypes.c_int,
+     [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32),
+      ctypes.POINTER(ctypes.c_float)]),
     ("wt_autotune_save", ctypes.c_int, [_vp, ctypes.c_char_p]),
     ("wt_autotune_load", ctypes.c_int, [_vp, ctypes.c_char_p]),
     ("wt_graph_begin", ctypes.c_int, [_vp, _vp]),
