@@ -120,29 +120,10 @@ __device__ __forceinline__ HmTap heightmap_fetch(const void* __restrict__ texels
     h.t11 = (float)p[r1 + c1];
   } else {
     const float* p = (const float*)texels;
-    if (W >= 2) {
-      // Both columns of a row of the footprint with one 8-byte load: they are adjacent
-      // (or equal, at a mirror fold), so the pair at min(c0, c1), kept below W - 1,
-      // holds them. Only across the REPEAT seam (c0 = W - 1, c1 = 0) is c0 outside the
-      // pair; those lanes load it on their own.
-      const int base = min(min(c0, c1), W - 2);
-      float2 a, b;
-      __builtin_memcpy(&a, p + r0 + base, sizeof(float2));
-      __builtin_memcpy(&b, p + r1 + base, sizeof(float2));
-      h.t00 = c0 == base ? a.x : a.y;
-      h.t01 = c0 == base ? b.x : b.y;
-      h.t10 = c1 == base ? a.x : a.y;
-      h.t11 = c1 == base ? b.x : b.y;
-      if (!tex_mirror(M) && c0 > base + 1) {
-        h.t00 = p[r0 + c0];
-        h.t01 = p[r1 + c0];
-      }
-    } else {
-      h.t00 = p[r0 + c0];
-      h.t10 = p[r0 + c1];
-      h.t01 = p[r1 + c0];
-      h.t11 = p[r1 + c1];
-    }
+    h.t00 = p[r0 + c0];
+    h.t10 = p[r0 + c1];
+    h.t01 = p[r1 + c0];
+    h.t11 = p[r1 + c1];
   }
   return h;
 }
