@@ -56,6 +56,7 @@ int main(void) {
   O(wt_patch, heightmapScrollVelocity)
   S(wt_fit_problem) O(wt_fit_problem, spacing) O(wt_fit_problem, target) O(wt_fit_problem, params)
   S(wt_adam_config)
+  S(wt_varying_buffers) O(wt_varying_buffers, ndc_xy_height0) O(wt_varying_buffers, heightmap_sample)
   return 0;
 }
 """
@@ -71,7 +72,7 @@ def test_struct_layouts_match_header(tmp_path):
     m = {"wt_gerstner_wave": _abi.GerstnerWave, "wt_camera": _abi.Camera, "wt_heightmap": _abi.Heightmap,
          "wt_water_params": _abi.WaterParams, "wt_vertex_buffers": _abi.VertexBuffers,
          "wt_launch_config": _abi.LaunchConfig, "wt_patch": _abi.Patch, "wt_fit_problem": _abi.FitProblem,
-         "wt_adam_config": _abi.AdamConfig}
+         "wt_adam_config": _abi.AdamConfig, "wt_varying_buffers": _abi.VaryingBuffers}
     for key, val in got.items():
         if "." in key:
             t, f = key.split(".")
@@ -104,6 +105,12 @@ def test_host_entry_points_reject_bad_arguments():
     assert lib.wt_water_displace(None, None, None, 0, None, ctypes.byref(vb), None) == wt.WT_ERR_INVALID_ARG
     assert lib.wt_fit_create(None, None, 0, None) == wt.WT_ERR_INVALID_ARG
     assert lib.wt_batch_create(None, None, 0, None) == wt.WT_ERR_INVALID_ARG
+    # the device-side entry points reject a missing context before touching the GPU
+    assert lib.wt_projector_corners_batched(None, None, None, 1, None, None, None) == wt.WT_ERR_INVALID_ARG
+    assert lib.wt_batch_set_cameras(None, None, None, None, None, None) == wt.WT_ERR_INVALID_ARG
+    assert lib.wt_grid_triangle_indices(None, 16, 0, None, None) == wt.WT_ERR_INVALID_ARG
+    assert lib.wt_autotune_batch(None, 64, 8, 1, None, None) == wt.WT_ERR_INVALID_ARG
+    assert lib.wt_vertex_varyings(None, None, None, None, None, None, None, 0, None, None) == wt.WT_ERR_INVALID_ARG
 
 
 def test_phase_plan():
