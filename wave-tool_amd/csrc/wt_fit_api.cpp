@@ -2,6 +2,7 @@
 // gradient evaluation, target synthesis and the phase-storage policy.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -88,7 +89,7 @@ wt_status wt_fit_create(wt_ctx* ctx, const wt_fit_problem* problems, uint32_t co
     d.S = (int)p.snapshots;
     d.W = (int)p.wave_count;
     d.nblocks = nblocks;
-    d.block_offset = class_blocks[cls];  // made global below, once the class sizes are known
+    d.block_offset = 0;  // set below, once the class sizes are known
     d.row_offset = f->total_rows;
     d.param_out_offset = f->total_params;
     class_blocks[cls] += nblocks;
@@ -108,7 +109,18 @@ wt_status wt_fit_create(wt_ctx* ctx, const wt_fit_problem* problems, uint32_t co
     f->launch.class_begin[c] = class_begin[c];
     f->launch.class_blocks[c] = class_blocks[c];
   }
-  for (uint32_t i = 0; i < count; ++i) f->host[i].block_offset += class_begin[wt::fit_class(f->host[i].S)];
+  // within a class, problems with more waves (more work per block) come first: the
+  // dispatcher starts blocks in index order, so the costliest blocks start earliest
+  std::vector<uint32_t> order(count);
+  for (uint32_t i = 0; i < count; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return f->host[a].W > f->host[b].W; });
+  long long class_next[wt::kFitClasses] = {};
+  for (uint32_t i : order) {
+    wt::FitDesc& d = f->host[i];
+    const int cls = wt::fit_class(d.S);
+    d.block_offset = class_begin[cls] + class_next[cls];
+    class_next[cls] += d.nblocks;
+  }
   if (f->total_blocks >= (1ll << 31) || f->total_rows >= (1ll << 31)) {
     delete f;
     return wt::ctx_fail(ctx, WT_ERR_UNSUPPORTED, "batch too large for one launch");
