@@ -28,6 +28,8 @@
 // table of one.
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 #include "wt_fit.hpp"
 
 namespace wt {
@@ -426,23 +428,50 @@ hipError_t launch_zero_words(uint32_t* p, size_t n, hipStream_t st) {
   return hipGetLastError();
 }
 
-// The adjoint launches of every snapshot class present, class C onwards.
+// The adjoint launch of snapshot class c on stream st.
 template <int C>
-hipError_t launch_adjoint_classes(const FitLaunch& L, int bump, hipStream_t st) {
-  if (L.class_blocks[C] > 0) {
+hipError_t launch_adjoint_class(const FitLaunch& L, int c, int bump, hipStream_t st) {
+  if (c == C) {
     hipLaunchKernelGGL((adjoint_kernel<kFitClassMaxS[C], kFitPointsPerThread[C]>), dim3((unsigned)L.class_blocks[C]),
                        dim3(kFitThreads), 0, st, L.probs, L.block_prob, L.class_begin[C], bump);
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    return hipGetLastError();
   }
-  if constexpr (C + 1 < kFitClasses) return launch_adjoint_classes<C + 1>(L, bump, st);
-  return hipSuccess;
+  if constexpr (C + 1 < kFitClasses) return launch_adjoint_class<C + 1>(L, c, bump, st);
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_fit_iteration(const FitLaunch& L, const FitAdam& adam, int mode, float* grad_out, float* loss_out,
                                 hipStream_t st) {
-  const hipError_t e = launch_adjoint_classes<0>(L, mode == 1 ? 1 : 0, st);
-  if (e != hipSuccess) return e;
+  const int bump = mode == 1 ? 1 : 0;
+  // classes present, the most work (blocks x snapshots) first
+  int order[kFitClasses], n = 0;
+  for (int c = 0; c < kFitClasses; ++c)
+    if (L.class_blocks[c] > 0) order[n++] = c;
+  auto work = [&](int c) { return (double)L.class_blocks[c] * kFitClassMaxS[c]; };
+  for (int i = 1; i < n; ++i)
+    for (int j = i; j > 0 && work(order[j]) > work(order[j - 1]); --j) std::swap(order[j], order[j - 1]);
+  if (n <= 1 || !L.side[0]) {
+    for (int i = 0; i < n; ++i) {
+      const hipError_t e = launch_adjoint_class<0>(L, order[i], bump, st);
+      if (e != hipSuccess) return e;
+    }
+  } else {
+    // class i on the caller's stream (i = 0) or side stream (i - 1) mod kFitSideStreams;
+    // the classes write disjoint partials and step counters, so they may overlap
+    hipError_t e = hipEventRecord(L.fork, st);
+    if (e != hipSuccess) return e;
+    const int used = n - 1 < kFitSideStreams ? n - 1 : kFitSideStreams;
+    for (int j = 0; j < used; ++j)
+      if ((e = hipStreamWaitEvent(L.side[j], L.fork, 0)) != hipSuccess) return e;
+    for (int i = 0; i < n; ++i)
+      if ((e = launch_adjoint_class<0>(L, order[i], bump, i == 0 ? st : L.side[(i - 1) % kFitSideStreams])) !=
+          hipSuccess)
+        return e;
+    for (int j = 0; j < used; ++j) {
+      if ((e = hipEventRecord(L.join[j], L.side[j])) != hipSuccess) return e;
+      if ((e = hipStreamWaitEvent(st, L.join[j], 0)) != hipSuccess) return e;
+    }
+  }
   hipLaunchKernelGGL(reduce_adam_kernel, dim3((unsigned)L.total_rows), dim3(kFitThreads), 0, st, L.probs, L.row_prob,
                      adam, mode, grad_out, loss_out);
   return hipGetLastError();

@@ -43,6 +43,11 @@ struct FitAdam {
 
 // Device tables of a batch: problems, block -> problem (blocks grouped by snapshot
 // class, class c at [class_begin[c], + class_blocks[c])), reduce row -> problem.
+// Streams beside the caller's on which the snapshot classes' adjoint launches run
+// concurrently (one class's tail overlaps the next class's blocks), joined by events
+// before the reduction.
+constexpr int kFitSideStreams = 3;
+
 struct FitLaunch {
   const FitDesc* probs;
   const int* block_prob;
@@ -50,6 +55,9 @@ struct FitLaunch {
   long long class_begin[kFitClasses];
   long long class_blocks[kFitClasses];
   long long total_rows;
+  hipStream_t side[kFitSideStreams];  // null: every class on the caller's stream
+  hipEvent_t fork;
+  hipEvent_t join[kFitSideStreams];
 };
 
 // mode 0: gradient + loss only (grad_out receives the concatenated gradients);

@@ -165,12 +165,33 @@ wt_status wt_fit_create(wt_ctx* ctx, const wt_fit_problem* problems, uint32_t co
   f->launch.block_prob = f->d_maps;
   f->launch.row_prob = f->d_maps + f->total_blocks;
   f->launch.total_rows = f->total_rows;
+  // side streams (default priority, not synchronising with the null stream) and their
+  // fork / join events, for batches spanning several snapshot classes
+  int classes = 0;
+  for (int c = 0; c < wt::kFitClasses; ++c) classes += f->launch.class_blocks[c] > 0;
+  if (classes > 1) {
+    e = hipEventCreateWithFlags(&f->launch.fork, hipEventDisableTiming);
+    for (int j = 0; j < wt::kFitSideStreams && e == hipSuccess; ++j) {
+      e = hipStreamCreateWithFlags(&f->launch.side[j], hipStreamNonBlocking);
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&f->launch.join[j], hipEventDisableTiming);
+    }
+    if (e != hipSuccess) {
+      wt_status s = hip_err(ctx, e, "wt_fit_create streams");
+      wt_fit_destroy(f);
+      return s;
+    }
+  }
   *out = f;
   return WT_OK;
 }
 
 void wt_fit_destroy(wt_fit* f) {
   if (!f) return;
+  for (int j = 0; j < wt::kFitSideStreams; ++j) {
+    if (f->launch.side[j]) (void)hipStreamDestroy(f->launch.side[j]);
+    if (f->launch.join[j]) (void)hipEventDestroy(f->launch.join[j]);
+  }
+  if (f->launch.fork) (void)hipEventDestroy(f->launch.fork);
   if (f->d_desc) (void)hipFree(f->d_desc);
   if (f->d_maps) (void)hipFree(f->d_maps);
   if (f->d_state) (void)hipFree(f->d_state);
