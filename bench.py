@@ -11,14 +11,16 @@ forward kernel (algorithmic 26 FLOP per vertex-wave over the mean launch time fr
 events on the launch stream, against the 157.3 TFLOP/s FP32 peak), the CPU baseline
 (the oracle's FP32 instantiation over std::threads on a bounded frame sample of the
 same workload), and `extras` for the other workloads: the batched forward over 512
-uneven patches, one heightfield fit, and 256 uneven fits batched against the same fits
-launched one by one.
+uneven patches, the varyings pass, the fragment stage on a 3840 x 2160 frame, one
+heightfield fit, and 256 uneven fits batched against the same fits launched one by one.
 """
 import argparse
 import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 for _p in (os.path.join(ROOT, "wave-tool_amd"), os.path.join(ROOT, "oracle")):
@@ -258,6 +260,56 @@ def bench_varyings(torch, wt, scenes, launches=100):
                               frac=round(traffic / t / 8e12, 4), bytes_per_vertex=76))
 
 
+def bench_shade(torch, wt, scenes, launches=50, cpu=True):
+    """The fragment stage (water-grid.frag as a deferred pass) on a 3840 x 2160 frame of
+    synthetic sea with full-resolution local reflection / refraction targets and 2048^2
+    skybox faces (CUBEMAP_LENGTH of src/render-engine.h:191), timed as a captured graph of
+    `launches` launches. HBM-bound: per covered pixel the 48 B fragment record, a 4 B
+    depth texel, one 4 B texel's worth of each of the three filtered lookups and the
+    16 B colour (80 B), plus the 1 B coverage byte of every pixel."""
+    import oracle as orc
+    dev = torch.device("cuda:0")
+    W, H, L = 3840, 2160, 2048
+    sc = scenes.shading_scene(W, H, seed=11, sky_length=L)
+    ctx = wt.Context()
+    t = {k: torch.from_numpy(sc[k]).to(dev) for k in ("depth", "reflections", "refractions", "skybox", "fragments",
+                                                         "coverage")}
+    colour = torch.empty((H, W, 4), device=dev)
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    ctx.graph_begin(stream)
+    for _ in range(launches):
+        ctx.shade(sc["uniforms"], t["depth"], t["reflections"], t["refractions"], t["skybox"], t["fragments"], colour,
+                  coverage=t["coverage"], stream=stream)
+    graph = ctx.graph_end(stream)
+    graph.launch(stream)
+    _, ev = timed(torch, stream, lambda: graph.launch(stream))
+    graph.close()
+    ctx.close()
+    tl = ev / launches
+    covered = int(sc["coverage"].sum())
+    traffic = 80 * covered + W * H
+    out = dict(workload="water fragment stage, 3840 x 2160 frame (%d covered pixels), local targets 3840 x 2160, "
+                        "skybox 6 x 2048^2 RGBA8" % covered,
+               value=covered / tl, unit="fragments/s", launch_us=round(tl * 1e6, 3),
+               roofline=dict(bound="hbm", achieved=round(traffic / tl / 1e9, 1), peak=8000.0, unit="GB/s",
+                             frac=round(traffic / tl / 8e12, 4), bytes_per_covered_pixel=80))
+    if cpu:
+        threads = cpu_threads()
+        sd = orc.ShadeData(sc["uniforms"], sc["depth"], sc["reflections"], sc["refractions"], sc["skybox"])
+        buf = np.empty((H, W, 4), dtype=np.float32)
+        frames, tc = 0, 0.0
+        while frames < 50 and tc < 3.0:
+            t0 = time.perf_counter()
+            orc.water_shade(sd, sc["fragments"], sc["coverage"], precision="f32", nthreads=threads, out=buf)
+            tc += time.perf_counter() - t0
+            frames += 1
+        out["cpu_baseline"] = dict(value=covered * frames / tc, unit="fragments/s", cores=threads, kind="port",
+                                   sample="%d whole frame(s), FP32 oracle, %d threads, %.2f s wall" % (frames, threads,
+                                                                                                      tc))
+    return out
+
+
 # --------------------------------------------------------------------- configs 3 and 4
 def make_fits(torch, wt, scenes, ctx, specs, sigma=0.01):
     """Device targets (synthesized from the hidden waves) and perturbed initial params."""
@@ -364,6 +416,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--launch-shape", default="", help="SPLIT,STEPS,LANES: fixed forward launch shape, no autotune")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--shade-only", action="store_true",
+                    help="run only the fragment-stage workload and print its line (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (requested --gpus %d)" % args.gpus,
@@ -376,6 +430,9 @@ def main():
     from wavetool_amd import scenes
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    if args.shade_only:
+        print(json.dumps(bench_shade(torch, wt, scenes, cpu=not args.no_cpu_baseline)), flush=True)
+        return
     out, inputs = bench_plain(torch, wt, scenes, args)
     log("config1 %.4g vertex-waves/s, %.2f us per frame" % (out["value"], out["roofline"]["launch_us"]))
     if not args.no_cpu_baseline:
@@ -387,6 +444,9 @@ def main():
         extras["varyings"] = bench_varyings(torch, wt, scenes)
         log("varyings %.4g vertices/s, %.1f GB/s" % (extras["varyings"]["value"],
                                                       extras["varyings"]["roofline"]["achieved"]))
+        extras["shade"] = bench_shade(torch, wt, scenes, cpu=not args.no_cpu_baseline)
+        log("fragment stage %.4g fragments/s, %.1f GB/s" % (extras["shade"]["value"],
+                                                            extras["shade"]["roofline"]["achieved"]))
         extras["fit_single"] = bench_fit_single(torch, wt, scenes, cpu=not args.no_cpu_baseline)
         log("config3 %.4g fits/s" % extras["fit_single"]["value"])
         extras["fit_batched"] = bench_fit_batched(torch, wt, scenes)

@@ -253,6 +253,72 @@ wt_status wt_autotune_load(wt_ctx* ctx, const char* path);
 wt_status wt_grid_triangle_indices(wt_ctx* ctx, uint32_t grid_length, uint32_t base_vertex, uint32_t* d_indices,
                                    void* stream);
 
+/* ---------------------------------------------------------------- fragment stage */
+/* One fragment of the water grid as the rasteriser hands it to the fragment shader
+ * (assets/shaders/water-grid.frag:57-61 plus gl_FragCoord.z): the varyings
+ * wt_vertex_varyings writes, interpolated to the pixel. 48 bytes; a fragment buffer
+ * holds one per pixel, rows bottom to top as GL numbers window rows, 16-byte aligned. */
+typedef struct wt_water_fragment {
+  float normal[3];               /* `normal`, interpolated (not renormalised, as the shader reads it) */
+  float view_vector[3];          /* viewVecRaw */
+  float normal_view_only_yaw[3]; /* normalVecInViewSpaceOnlyYaw */
+  float ndc_xy_height0[2];       /* xyPositionNDCSpaceHeight0 */
+  float frag_depth;              /* gl_FragCoord.z, window-space depth in [0, 1] */
+} wt_water_fragment;
+
+/* The fragment shader's uniforms (water-grid.frag:34-55), GLSL names, as
+ * RenderEngine::render sets them (src/render-engine.cpp:1060-1098; defaults in
+ * src/render-engine.h:213-235). */
+typedef struct wt_shade_params {
+  float fogColourFarAtCurrentTime[4];
+  float fogDepthRadiusFar;
+  float fogDepthRadiusNear;
+  float softEdgesDeltaDepthThreshold;
+  float sunPosition[3];
+  float sunShininess; /* >= 0 */
+  float sunStrength;
+  float tintDeltaDepthThreshold;
+  float viewportWidthHeight[2];
+  float waterClarity;
+  float zFar; /* zFar > zNear > 0 */
+  float zNear;
+} wt_shade_params;
+
+/* An RGBA8 image in device memory (GL_RGBA / GL_UNSIGNED_BYTE, 4-byte aligned), rows
+ * bottom to top; sides 1..32768. */
+typedef struct wt_rgba8_image {
+  const uint8_t* texels;
+  uint32_t width, height;
+} wt_rgba8_image;
+
+/* The shader's samplers with the reference's filtering (src/render-engine.cpp:90-188):
+ * depthTexture2D (one float per texel, GL_NEAREST), localReflectionsTexture2D and
+ * localRefractionsTexture2D (GL_LINEAR), skybox (six skybox_length^2 RGBA8 faces back
+ * to back in GL target order +X, -X, +Y, -Y, +Z, -Z, GL_LINEAR per face), all
+ * CLAMP_TO_EDGE. */
+typedef struct wt_shade_textures {
+  const float* depth;
+  uint32_t depth_width, depth_height;
+  wt_rgba8_image local_reflections;
+  wt_rgba8_image local_refractions;
+  const uint8_t* skybox;
+  uint32_t skybox_length;
+} wt_shade_textures;
+
+/* The water program's fragment stage (water-grid.frag:70-162) as a deferred pass over
+ * a width x height fragment buffer (sides 1..32768): for every pixel whose coverage
+ * byte is nonzero (coverage NULL: every pixel), the fragment colour RGBA (float4, 16-byte
+ * aligned colour buffer) with gl_FragCoord.xy at the pixel centre. blend = 0 writes the
+ * colour; blend = 1 blends it over the colour buffer's value as the reference's draw
+ * does (glBlendFunc(GL_SRC_ALPHA, GL_ONE_MINUS_SRC_ALPHA), src/render-engine.cpp:
+ * 302-303, on all four channels of a floating-point buffer). Uncovered pixels are not
+ * touched. Replaces the fragment half of the glDrawElements at
+ * src/render-engine.cpp:1115; the rasterisation that produces the fragments stays
+ * with the renderer. Stream-ordered and capturable. */
+wt_status wt_water_shade(wt_ctx* ctx, const wt_shade_params* params, const wt_shade_textures* textures,
+                         const wt_water_fragment* fragments, const uint8_t* coverage, uint32_t width,
+                         uint32_t height, float* colour, int blend, void* stream);
+
 /* ---------------------------------------------------------------- batched forward */
 /* One ocean patch of a batch: its own projected grid (corners from
  * wt_projector_corners of its own camera), time, scales and a run of the batch's

@@ -58,6 +58,34 @@ class Fit(ctypes.Structure):
     ]
 
 
+class Shade(ctypes.Structure):
+    _fields_ = [
+        ("fog", ctypes.c_double * 4),
+        ("fog_far", ctypes.c_double),
+        ("fog_near", ctypes.c_double),
+        ("soft_edges", ctypes.c_double),
+        ("sun", ctypes.c_double * 3),
+        ("shininess", ctypes.c_double),
+        ("strength", ctypes.c_double),
+        ("tint", ctypes.c_double),
+        ("viewport", ctypes.c_double * 2),
+        ("clarity", ctypes.c_double),
+        ("zfar", ctypes.c_double),
+        ("znear", ctypes.c_double),
+        ("depth", ctypes.c_void_p),
+        ("depth_w", ctypes.c_int32),
+        ("depth_h", ctypes.c_int32),
+        ("refl", ctypes.c_void_p),
+        ("refl_w", ctypes.c_int32),
+        ("refl_h", ctypes.c_int32),
+        ("refr", ctypes.c_void_p),
+        ("refr_w", ctypes.c_int32),
+        ("refr_h", ctypes.c_int32),
+        ("sky", ctypes.c_void_p),
+        ("sky_len", ctypes.c_int32),
+    ]
+
+
 _lib = None
 
 
@@ -94,6 +122,19 @@ def lib():
             f = getattr(L, name)
             f.argtypes = [ctypes.POINTER(Fit), dp, ctypes.c_int, dp, ctypes.c_int]
             f.restype = ctypes.c_double
+        for name, ptr in (("wto_water_shade_f64", dp), ("wto_water_shade_f32", fp)):
+            f = getattr(L, name)
+            f.argtypes = [ctypes.POINTER(Shade), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong,
+                          ctypes.c_longlong, ctypes.c_longlong, ptr, ptr, ctypes.c_int]
+            f.restype = ctypes.c_int
+        L.wto_cube_face.argtypes = [dp, dp, dp]
+        L.wto_cube_face.restype = ctypes.c_int
+        L.wto_sample_rgba8.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                       dp]
+        L.wto_sample_cube.argtypes = [ctypes.c_void_p, ctypes.c_int, dp, dp]
+        L.wto_sample_depth.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                       ctypes.c_double]
+        L.wto_sample_depth.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -221,3 +262,87 @@ def fit_loss_grad(fd, params, nthreads=1, precision="f64", grad=True):
     fn = lib().wto_fit_loss_grad_f64 if precision == "f64" else lib().wto_fit_loss_grad_f32
     loss = fn(ctypes.byref(fd.fit), _p(p), W, _p(g), nthreads)
     return loss, g
+
+
+# ---------------------------------------------------------------- fragment stage
+SHADE_UNIFORMS = ("fogColourFarAtCurrentTime", "fogDepthRadiusFar", "fogDepthRadiusNear",
+                  "softEdgesDeltaDepthThreshold", "sunPosition", "sunShininess", "sunStrength",
+                  "tintDeltaDepthThreshold", "viewportWidthHeight", "waterClarity", "zFar", "zNear")
+
+
+class ShadeData:
+    """The fragment shader's uniforms (dict keyed by the GLSL names) and textures:
+    depth (h, w) float32, reflections / refractions (h, w, 4) uint8, skybox
+    (6, L, L, 4) uint8. Keeps the arrays the Shade struct points to alive."""
+
+    def __init__(self, uniforms, depth, reflections, refractions, skybox):
+        self.depth = np.ascontiguousarray(depth, dtype=np.float32)
+        self.refl = np.ascontiguousarray(reflections, dtype=np.uint8)
+        self.refr = np.ascontiguousarray(refractions, dtype=np.uint8)
+        self.sky = np.ascontiguousarray(skybox, dtype=np.uint8)
+        u = uniforms
+        s = Shade()
+        s.fog[:] = [float(v) for v in u["fogColourFarAtCurrentTime"]]
+        s.fog_far, s.fog_near = float(u["fogDepthRadiusFar"]), float(u["fogDepthRadiusNear"])
+        s.soft_edges = float(u["softEdgesDeltaDepthThreshold"])
+        s.sun[:] = [float(v) for v in u["sunPosition"]]
+        s.shininess, s.strength = float(u["sunShininess"]), float(u["sunStrength"])
+        s.tint = float(u["tintDeltaDepthThreshold"])
+        s.viewport[:] = [float(v) for v in u["viewportWidthHeight"]]
+        s.clarity, s.zfar, s.znear = float(u["waterClarity"]), float(u["zFar"]), float(u["zNear"])
+        s.depth = self.depth.ctypes.data
+        s.depth_h, s.depth_w = self.depth.shape
+        s.refl = self.refl.ctypes.data
+        s.refl_h, s.refl_w = self.refl.shape[:2]
+        s.refr = self.refr.ctypes.data
+        s.refr_h, s.refr_w = self.refr.shape[:2]
+        s.sky = self.sky.ctypes.data
+        s.sky_len = self.sky.shape[1]
+        self.shade = s
+
+
+def water_shade(sd, fragments, coverage=None, dst=None, precision="f64", nthreads=1, first=0, count=None, out=None):
+    """Fragment colours of a (h, w, 12) float32 fragment buffer; returns (h, w, 4).
+    Pixels outside [first, first + count) and uncovered pixels keep out's values (dst's
+    with blending, NaN otherwise)."""
+    frags = np.ascontiguousarray(fragments, dtype=np.float32)
+    h, w = frags.shape[:2]
+    cov = None if coverage is None else np.ascontiguousarray(coverage, dtype=np.uint8)
+    dt = np.float32 if precision == "f32" else np.float64
+    ct = ctypes.c_float if precision == "f32" else ctypes.c_double
+    d = None if dst is None else np.ascontiguousarray(dst, dtype=dt)
+    if out is None:
+        out = d.copy() if d is not None else np.full((h, w, 4), np.nan, dtype=dt)
+    if count is None:
+        count = h * w - first
+    fn = lib().wto_water_shade_f64 if precision == "f64" else lib().wto_water_shade_f32
+    rc = fn(ctypes.byref(sd.shade), frags.ctypes.data, None if cov is None else cov.ctypes.data, w, first, count,
+            _p(out, ct), _p(d, ct), nthreads)
+    if rc != 0:
+        raise RuntimeError("oracle water_shade failed")
+    return out
+
+
+def cube_face(r):
+    s, t = ctypes.c_double(), ctypes.c_double()
+    f = lib().wto_cube_face(_p(np.ascontiguousarray(r, dtype=np.float64)), ctypes.byref(s), ctypes.byref(t))
+    return f, s.value, t.value
+
+
+def sample_rgba8(texels, s, t):
+    tex = np.ascontiguousarray(texels, dtype=np.uint8)
+    out = np.zeros(4)
+    lib().wto_sample_rgba8(tex.ctypes.data, tex.shape[1], tex.shape[0], float(s), float(t), _p(out))
+    return out
+
+
+def sample_cube(faces, r):
+    f = np.ascontiguousarray(faces, dtype=np.uint8)
+    out = np.zeros(4)
+    lib().wto_sample_cube(f.ctypes.data, f.shape[1], _p(np.ascontiguousarray(r, dtype=np.float64)), _p(out))
+    return out
+
+
+def sample_depth(texels, s, t):
+    tex = np.ascontiguousarray(texels, dtype=np.float32)
+    return lib().wto_sample_depth(tex.ctypes.data, tex.shape[1], tex.shape[0], float(s), float(t))

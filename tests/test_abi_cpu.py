@@ -57,6 +57,11 @@ int main(void) {
   S(wt_fit_problem) O(wt_fit_problem, spacing) O(wt_fit_problem, target) O(wt_fit_problem, params)
   S(wt_adam_config)
   S(wt_varying_buffers) O(wt_varying_buffers, ndc_xy_height0) O(wt_varying_buffers, heightmap_sample)
+  S(wt_shade_params) O(wt_shade_params, sunPosition) O(wt_shade_params, viewportWidthHeight) O(wt_shade_params, zNear)
+  S(wt_rgba8_image) O(wt_rgba8_image, height)
+  S(wt_shade_textures) O(wt_shade_textures, local_reflections) O(wt_shade_textures, local_refractions)
+  O(wt_shade_textures, skybox) O(wt_shade_textures, skybox_length)
+  S(wt_water_fragment) O(wt_water_fragment, ndc_xy_height0) O(wt_water_fragment, frag_depth)
   return 0;
 }
 """
@@ -72,8 +77,15 @@ def test_struct_layouts_match_header(tmp_path):
     m = {"wt_gerstner_wave": _abi.GerstnerWave, "wt_camera": _abi.Camera, "wt_heightmap": _abi.Heightmap,
          "wt_water_params": _abi.WaterParams, "wt_vertex_buffers": _abi.VertexBuffers,
          "wt_launch_config": _abi.LaunchConfig, "wt_patch": _abi.Patch, "wt_fit_problem": _abi.FitProblem,
-         "wt_adam_config": _abi.AdamConfig, "wt_varying_buffers": _abi.VaryingBuffers}
+         "wt_adam_config": _abi.AdamConfig, "wt_varying_buffers": _abi.VaryingBuffers,
+         "wt_shade_params": _abi.ShadeParams, "wt_rgba8_image": _abi.Rgba8Image,
+         "wt_shade_textures": _abi.ShadeTextures, "wt_water_fragment": None}
+    # the fragment record: 12 floats, read by the kernel as three float4
+    assert got["wt_water_fragment"] == str(_abi.WT_BYTES_PER_FRAGMENT)
+    assert got["wt_water_fragment.ndc_xy_height0"] == "36" and got["wt_water_fragment.frag_depth"] == "44"
     for key, val in got.items():
+        if key.startswith("wt_water_fragment"):
+            continue
         if "." in key:
             t, f = key.split(".")
             assert getattr(m[t], f).offset == int(val), key
@@ -111,6 +123,7 @@ def test_host_entry_points_reject_bad_arguments():
     assert lib.wt_grid_triangle_indices(None, 16, 0, None, None) == wt.WT_ERR_INVALID_ARG
     assert lib.wt_autotune_batch(None, 64, 8, 1, None, None) == wt.WT_ERR_INVALID_ARG
     assert lib.wt_vertex_varyings(None, None, None, None, None, None, None, 0, None, None) == wt.WT_ERR_INVALID_ARG
+    assert lib.wt_water_shade(None, None, None, None, None, 4, 4, None, 0, None) == wt.WT_ERR_INVALID_ARG
 
 
 def test_phase_plan():

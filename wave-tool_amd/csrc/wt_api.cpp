@@ -793,6 +793,89 @@ wt_status wt_grid_triangle_indices(wt_ctx* ctx, uint32_t grid_length, uint32_t b
   return WT_OK;
 }
 
+static bool side_ok(uint32_t v) { return v >= 1 && v <= 32768; }
+
+wt_status wt_water_shade(wt_ctx* ctx, const wt_shade_params* p, const wt_shade_textures* tx,
+                         const wt_water_fragment* fragments, const uint8_t* coverage, uint32_t width, uint32_t height,
+                         float* colour, int blend, void* stream) {
+  static_assert(sizeof(wt_water_fragment) == 48, "the kernel reads a fragment as three float4");
+  if (!ctx) return WT_ERR_INVALID_ARG;
+  if (!p || !tx || !fragments || !colour) return fail(ctx, WT_ERR_INVALID_ARG, "params, textures, fragments and colour must be non-NULL");
+  if (!side_ok(width) || !side_ok(height)) return fail(ctx, WT_ERR_INVALID_ARG, "width and height must be in [1, 32768]");
+  if (blend != 0 && blend != 1) return fail(ctx, WT_ERR_INVALID_ARG, "blend must be 0 or 1");
+  const float fs[20] = {p->fogColourFarAtCurrentTime[0], p->fogColourFarAtCurrentTime[1], p->fogColourFarAtCurrentTime[2],
+                        p->fogColourFarAtCurrentTime[3], p->fogDepthRadiusFar, p->fogDepthRadiusNear,
+                        p->softEdgesDeltaDepthThreshold, p->sunPosition[0], p->sunPosition[1], p->sunPosition[2],
+                        p->sunShininess, p->sunStrength, p->tintDeltaDepthThreshold, p->viewportWidthHeight[0],
+                        p->viewportWidthHeight[1], p->waterClarity, p->zFar, p->zNear, 0.0f, 0.0f};
+  for (float v : fs)
+    if (!std::isfinite(v)) return fail(ctx, WT_ERR_INVALID_ARG, "uniforms must be finite");
+  if (!(p->zFar > p->zNear && p->zNear > 0.0f)) return fail(ctx, WT_ERR_INVALID_ARG, "need zFar > zNear > 0");
+  if (!(p->viewportWidthHeight[0] > 0.0f && p->viewportWidthHeight[1] > 0.0f))
+    return fail(ctx, WT_ERR_INVALID_ARG, "viewportWidthHeight must be positive");
+  if (p->sunShininess < 0.0f) return fail(ctx, WT_ERR_INVALID_ARG, "sunShininess must be >= 0");
+  if (!tx->depth || !tx->local_reflections.texels || !tx->local_refractions.texels || !tx->skybox)
+    return fail(ctx, WT_ERR_INVALID_ARG, "every texture is required");
+  if (!side_ok(tx->depth_width) || !side_ok(tx->depth_height) || !side_ok(tx->local_reflections.width) ||
+      !side_ok(tx->local_reflections.height) || !side_ok(tx->local_refractions.width) ||
+      !side_ok(tx->local_refractions.height) || !side_ok(tx->skybox_length))
+    return fail(ctx, WT_ERR_INVALID_ARG, "texture sides must be in [1, 32768]");
+  if ((reinterpret_cast<uintptr_t>(fragments) & 15) || (reinterpret_cast<uintptr_t>(colour) & 15))
+    return fail(ctx, WT_ERR_INVALID_ARG, "fragments and colour must be 16-byte aligned");
+  const void* words[4] = {tx->depth, tx->local_reflections.texels, tx->local_refractions.texels, tx->skybox};
+  for (const void* b : words)
+    if (reinterpret_cast<uintptr_t>(b) & 3) return fail(ctx, WT_ERR_INVALID_ARG, "textures must be 4-byte aligned");
+  hipStream_t st = (hipStream_t)stream;
+  if (!stream_capturing(st)) {
+    const void* bufs[7] = {fragments, colour, tx->depth, tx->local_reflections.texels, tx->local_refractions.texels,
+                           tx->skybox, coverage};
+    for (const void* b : bufs)
+      if (b && !is_device_pointer(b)) return fail(ctx, WT_ERR_INVALID_ARG, "buffers must be device memory");
+  }
+  // FP32 uniforms and the reciprocals the kernel multiplies by in place of the shader's
+  // divisions (a reciprocal-multiply is within GLSL's 2.5 ulp for division)
+  wt::ShadeArgs a{};
+  for (int k = 0; k < 4; ++k) a.fog[k] = p->fogColourFarAtCurrentTime[k];
+  a.fog_far = p->fogDepthRadiusFar;
+  a.fog_near = p->fogDepthRadiusNear;
+  a.inv_fog_range = p->fogDepthRadiusFar > p->fogDepthRadiusNear
+                        ? 1.0f / (p->fogDepthRadiusFar - p->fogDepthRadiusNear) : 0.0f;
+  a.tint_off = p->tintDeltaDepthThreshold == 0.0f;
+  a.inv_tint = a.tint_off ? 0.0f : 1.0f / p->tintDeltaDepthThreshold;
+  a.soft_off = p->softEdgesDeltaDepthThreshold == 0.0f;
+  a.inv_soft = a.soft_off ? 0.0f : 1.0f / p->softEdgesDeltaDepthThreshold;
+  for (int k = 0; k < 3; ++k) a.sun[k] = p->sunPosition[k];
+  a.shininess = p->sunShininess;
+  a.strength = p->sunStrength;
+  a.clarity = p->waterClarity;
+  a.zfar = p->zFar;
+  a.znear = p->zNear;
+  a.zfar_m_znear = p->zFar - p->zNear;
+  a.inv_zfar = 1.0f / p->zFar;
+  a.inv_viewport_w = 1.0f / p->viewportWidthHeight[0];
+  a.inv_viewport_h = 1.0f / p->viewportWidthHeight[1];
+  a.depth = tx->depth;
+  a.depth_w = (int)tx->depth_width;
+  a.depth_h = (int)tx->depth_height;
+  a.refl = reinterpret_cast<const unsigned*>(tx->local_reflections.texels);
+  a.refl_w = (int)tx->local_reflections.width;
+  a.refl_h = (int)tx->local_reflections.height;
+  a.refr = reinterpret_cast<const unsigned*>(tx->local_refractions.texels);
+  a.refr_w = (int)tx->local_refractions.width;
+  a.refr_h = (int)tx->local_refractions.height;
+  a.sky = reinterpret_cast<const unsigned*>(tx->skybox);
+  a.sky_len = (int)tx->skybox_length;
+  a.frags = reinterpret_cast<const float4*>(fragments);
+  a.coverage = coverage;
+  a.colour = reinterpret_cast<float4*>(colour);
+  a.width = (int)width;
+  a.count = (int)(width * height);
+  a.blend = blend;
+  hipError_t e = wt::launch_shade(a, st);
+  if (e != hipSuccess) return hip_fail(ctx, e, "water_shade_kernel launch");
+  return WT_OK;
+}
+
 wt_status wt_water_displace_batched(wt_ctx* ctx, wt_batch* b, const wt_gerstner_wave* d_waves, uint32_t total_waves,
                                     const wt_heightmap* hm, const wt_vertex_buffers* out, double time_offset,
                                     void* stream) {

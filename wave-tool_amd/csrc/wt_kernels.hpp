@@ -131,4 +131,32 @@ hipError_t launch_grid_indices(uint32_t n, uint32_t base, uint32_t* out, hipStre
 // unit of a.
 hipError_t launch_forward_batched(const BatchArgs& a, int shape, hipStream_t st);
 
+// Fragment stage (wt_water_shade): the water program's fragment shader over a
+// per-pixel fragment buffer. RGBA8 images are read as one 32-bit word per texel.
+struct ShadeArgs {
+  float fog[4];  // fogColourFarAtCurrentTime
+  float fog_far, fog_near;
+  float inv_fog_range;  // 1 / (fogDepthRadiusFar - fogDepthRadiusNear), used only between the radii
+  int tint_off, soft_off;   // tintDeltaDepthThreshold / softEdgesDeltaDepthThreshold == 0
+  float inv_tint, inv_soft;  // their reciprocals otherwise
+  float sun[3];
+  float shininess, strength, clarity, zfar, znear;
+  float zfar_m_znear, inv_zfar;
+  float inv_viewport_w, inv_viewport_h;  // 1 / viewportWidthHeight
+  const float* depth;
+  int depth_w, depth_h;
+  const unsigned* refl;
+  int refl_w, refl_h;
+  const unsigned* refr;
+  int refr_w, refr_h;
+  const unsigned* sky;  // six faces, sky_len^2 texels each
+  int sky_len;
+  const float4* frags;  // three float4 per pixel
+  const unsigned char* coverage;
+  float4* colour;
+  int width, count;  // count = width x height <= 2^30
+  int blend;
+};
+hipError_t launch_shade(const ShadeArgs& a, hipStream_t st);
+
 }  // namespace wt

@@ -9,13 +9,13 @@ import ctypes
 
 from . import scenes
 from ._abi import (LaunchConfig, Camera, GerstnerWave, Heightmap, VertexBuffers, VaryingBuffers, WaterParams,
-                   FitProblem, AdamConfig, Patch, WtError, check, lib,
+                   FitProblem, AdamConfig, Patch, ShadeParams, ShadeTextures, Rgba8Image, WtError, check, lib,
                    WT_OK, WT_ERR_INVALID_ARG, WT_ERR_NO_INTERSECTION, WT_ERR_DEGENERATE, WT_WRAP_REPEAT,
                    WT_WRAP_MIRRORED_REPEAT, WT_TEXEL_R8_UNORM, WT_TEXEL_R32F, WT_FLOP_PER_VERTEX_WAVE,
-                   WT_BYTES_PER_VERTEX_PN, WT_BYTES_PER_VERTEX_PNTB)
+                   WT_BYTES_PER_VERTEX_PN, WT_BYTES_PER_VERTEX_PNTB, WT_BYTES_PER_FRAGMENT)
 
-__all__ = ["Context", "camera", "projector_corners", "projector_corners_batched", "camera_rows", "water_params", "normalize_steepness", "scenes", "lib", "check",
-           "WtError"]
+__all__ = ["Context", "camera", "projector_corners", "projector_corners_batched", "camera_rows", "water_params", "normalize_steepness",
+           "shade_params", "scenes", "lib", "check", "WtError"]
 
 
 def camera(cam_tuple):
@@ -97,6 +97,18 @@ def _ptr(t):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
 
 
+def shade_params(uniforms):
+    """wt_shade_params from a dict keyed by the fragment shader's uniform names."""
+    p = ShadeParams()
+    for name, _ in ShadeParams._fields_:
+        v = uniforms[name]
+        if isinstance(v, (list, tuple)) or hasattr(v, "__len__"):
+            getattr(p, name)[:] = [float(x) for x in v]
+        else:
+            setattr(p, name, float(v))
+    return p
+
+
 def _check_array(t, dtype, numel, name):
     """The kernels index count entries of each array: refuse a tensor that holds fewer."""
     if t.dtype != dtype or not t.is_contiguous() or t.numel() != numel:
@@ -139,6 +151,34 @@ class Context:
         import torch
         _check_array(out, torch.int32, 6 * (int(n) - 1) ** 2, "indices")
         self._check(lib().wt_grid_triangle_indices(self.handle, int(n), int(base), _ptr(out), _stream(stream)))
+
+    def shade(self, uniforms, depth, reflections, refractions, skybox, fragments, colour, coverage=None, blend=False,
+              stream=None):
+        """The water fragment stage (wt_water_shade) on device tensors: depth (h, w)
+        float32, reflections / refractions (h, w, 4) uint8, skybox (6, L, L, 4) uint8,
+        fragments (H, W, 12) float32 (wt_water_fragment records), colour (H, W, 4)
+        float32, coverage (H, W) uint8 or None. uniforms: dict by GLSL name or a
+        ShadeParams."""
+        import torch
+        H, W = int(fragments.shape[0]), int(fragments.shape[1])
+        _check_array(fragments, torch.float32, H * W * 12, "fragments")
+        _check_array(colour, torch.float32, H * W * 4, "colour")
+        if coverage is not None:
+            _check_array(coverage, torch.uint8, H * W, "coverage")
+        dh, dw = int(depth.shape[0]), int(depth.shape[1])
+        _check_array(depth, torch.float32, dh * dw, "depth")
+        imgs = []
+        for name, t in (("reflections", reflections), ("refractions", refractions)):
+            h, w = int(t.shape[0]), int(t.shape[1])
+            _check_array(t, torch.uint8, h * w * 4, name)
+            imgs.append(Rgba8Image(t.data_ptr(), w, h))
+        L = int(skybox.shape[1])
+        _check_array(skybox, torch.uint8, 6 * L * L * 4, "skybox")
+        tx = ShadeTextures(depth.data_ptr(), dw, dh, imgs[0], imgs[1], skybox.data_ptr(), L)
+        p = uniforms if isinstance(uniforms, ShadeParams) else shade_params(uniforms)
+        self._check(lib().wt_water_shade(self.handle, ctypes.byref(p), ctypes.byref(tx), _ptr(fragments),
+                                         _ptr(coverage) if coverage is not None else None, W, H, _ptr(colour),
+                                         1 if blend else 0, _stream(stream)))
 
     def displace(self, params, waves, pos, nrm, tan=None, bit=None, heightmap=None, hm_format=WT_TEXEL_R32F,
                  hm_wrap=WT_WRAP_REPEAT, stream=None, wave_count=None):

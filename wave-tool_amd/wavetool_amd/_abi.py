@@ -144,6 +144,27 @@ class AdamConfig(ctypes.Structure):
                 ("eps", ctypes.c_float)]
 
 
+class ShadeParams(ctypes.Structure):
+    _fields_ = [("fogColourFarAtCurrentTime", ctypes.c_float * 4), ("fogDepthRadiusFar", ctypes.c_float),
+                ("fogDepthRadiusNear", ctypes.c_float), ("softEdgesDeltaDepthThreshold", ctypes.c_float),
+                ("sunPosition", ctypes.c_float * 3), ("sunShininess", ctypes.c_float), ("sunStrength", ctypes.c_float),
+                ("tintDeltaDepthThreshold", ctypes.c_float), ("viewportWidthHeight", ctypes.c_float * 2),
+                ("waterClarity", ctypes.c_float), ("zFar", ctypes.c_float), ("zNear", ctypes.c_float)]
+
+
+class Rgba8Image(ctypes.Structure):
+    _fields_ = [("texels", ctypes.c_void_p), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32)]
+
+
+class ShadeTextures(ctypes.Structure):
+    _fields_ = [("depth", ctypes.c_void_p), ("depth_width", ctypes.c_uint32), ("depth_height", ctypes.c_uint32),
+                ("local_reflections", Rgba8Image), ("local_refractions", Rgba8Image), ("skybox", ctypes.c_void_p),
+                ("skybox_length", ctypes.c_uint32)]
+
+
+WT_BYTES_PER_FRAGMENT = 48  # sizeof(wt_water_fragment)
+
+
 # (name, restype, argtypes) for every entry point declared in include/wt.h
 _vp = ctypes.c_void_p
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -184,6 +205,9 @@ PROTOTYPES = [
     ("wt_batch_patch_launch_config", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.POINTER(LaunchConfig)]),
     ("wt_batch_set_cameras", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     ("wt_grid_triangle_indices", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp]),
+    ("wt_water_shade", ctypes.c_int,
+     [_vp, ctypes.POINTER(ShadeParams), ctypes.POINTER(ShadeTextures), _vp, _vp, ctypes.c_uint32, ctypes.c_uint32, _vp,
+      ctypes.c_int, _vp]),
     ("wt_water_displace_batched", ctypes.c_int,
      [_vp, _vp, _vp, ctypes.c_uint32, ctypes.POINTER(Heightmap), ctypes.POINTER(VertexBuffers), ctypes.c_double, _vp]),
     ("wt_fit_create", ctypes.c_int, [_vp, ctypes.POINTER(FitProblem), ctypes.c_uint32, ctypes.POINTER(_vp)]),

@@ -234,3 +234,157 @@ def fit_target(prob, chunk=1 << 16):
         nz = np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * math.pi * u2)
         out += (prob["noise"] * nz).reshape(S, V, 3).astype(np.float32)
     return out
+
+
+# --------------------------------------------------------------------- fragment stage
+SHADE_TIME_OF_DAY_H = 9.0  # timeOfDayInHours default (src/render-engine.h:233)
+
+
+def shade_uniforms(width, height, time_of_day_h=SHADE_TIME_OF_DAY_H):
+    """The fragment shader's uniforms with the reference's defaults
+    (src/render-engine.h:213-235): sun from the time of day as at
+    src/render-engine.cpp:289-291, fog colour at :295 (all in float32)."""
+    f32 = np.float32
+    theta = f32(f32(time_of_day_h) / f32(24.0)) * f32(2.0 * math.pi) - f32(0.5 * math.pi)
+    sun = [float(f32(np.cos(theta))), float(f32(np.sin(theta))), 0.0]
+    k = float(f32(min(max(sun[1], 0.0), 1.0)))
+    return {"fogColourFarAtCurrentTime": [k, k, k, 0.1], "fogDepthRadiusFar": 1.0, "fogDepthRadiusNear": 0.0,
+            "softEdgesDeltaDepthThreshold": 0.05, "sunPosition": sun, "sunShininess": 50.0, "sunStrength": 1.0,
+            "tintDeltaDepthThreshold": 0.1, "viewportWidthHeight": [float(width), float(height)],
+            "waterClarity": 0.3, "zFar": 100.0, "zNear": 0.1}
+
+
+def cube_texel_directions(length, face=None):
+    """(6, L, L, 3) directions through the texel centres of a cube map's faces in GL
+    target order (or (L, L, 3) for one face), inverting the face table of the GL spec
+    (table 8.19): +X (1, -tc, -sc), -X (-1, -tc, sc), +Y (sc, 1, tc), -Y (sc, -1, -tc),
+    +Z (sc, -tc, 1), -Z (-sc, -tc, -1), with sc = 2 s - 1 along a row, tc = 2 t - 1."""
+    c = (np.arange(length) + 0.5) / length * 2.0 - 1.0
+    sc = np.broadcast_to(c[None, :], (length, length))
+    tc = np.broadcast_to(c[:, None], (length, length))
+    one = np.ones((length, length))
+    table = [lambda: (one, -tc, -sc), lambda: (-one, -tc, sc), lambda: (sc, one, tc), lambda: (sc, -one, -tc),
+             lambda: (sc, -tc, one), lambda: (-sc, -tc, -one)]
+    if face is not None:
+        return np.stack(table[face](), axis=-1)
+    return np.stack([np.stack(f(), axis=-1) for f in table])
+
+
+def _noise_rgba(seed, h, w, alpha):
+    """Smooth RGBA8 image: value noise per channel (bilinearly resampled), alpha from
+    `alpha` (a constant, or 'mask' for a half-covered pattern of 0 and 255 regions)."""
+    out = np.empty((h, w, 4), dtype=np.uint8)
+    size = 64
+    for ch in range(4 if alpha == "noise" else 3):
+        base = value_noise(seed * 7 + ch, size=size, octaves=4, base_cells=2).astype(np.float64)
+        ys = (np.arange(h) + 0.5) / h * size - 0.5
+        xs = (np.arange(w) + 0.5) / w * size - 0.5
+        y0 = np.floor(ys).astype(np.int64)
+        x0 = np.floor(xs).astype(np.int64)
+        fy, fx = ys - y0, xs - x0
+        y0m, y1m, x0m, x1m = y0 % size, (y0 + 1) % size, x0 % size, (x0 + 1) % size
+        top = base[np.ix_(y0m, x0m)] * (1 - fx) + base[np.ix_(y0m, x1m)] * fx
+        bot = base[np.ix_(y1m, x0m)] * (1 - fx) + base[np.ix_(y1m, x1m)] * fx
+        v = top * (1 - fy)[:, None] + bot * fy[:, None]
+        out[..., ch] = np.clip(np.rint(v * 255.0), 0, 255).astype(np.uint8)
+    if alpha == "mask":
+        out[..., 3] = np.where(out[..., 0] > 128, 255, 0).astype(np.uint8)
+    elif alpha != "noise":
+        out[..., 3] = int(alpha)
+    return out
+
+
+def shading_scene(width, height, seed=0, sky_length=256, local_scale=1.0, eye_height=6.0, pitch_deg=-12.0,
+                  yaw_deg=30.0, fov_deg=60.0):
+    """A synthetic water frame for the fragment stage: the fragment buffer a rasteriser
+    would produce for a rippled sea seen from a camera `eye_height` m up, plus the
+    textures the shader samples.
+
+    Per pixel the camera ray meets the plane y = 0 at P; the surface height and normal
+    come from a sum of sines; `normal` is shortened by up to 2% (interpolation);
+    viewVecRaw = eye - P; the yaw-only view normal is the normal rotated by -yaw about y;
+    xyPositionNDCSpaceHeight0 projects (P.x, 0, P.z); gl_FragCoord.z is P's window depth.
+    Pixels whose ray misses the sea or runs past zFar are not covered. The depth texture
+    holds a sea bed 1-6 m down (1.0 = sky where no bed is hit), reflections / refractions
+    are smooth RGBA8 noise at local_scale x the viewport, the skybox a smooth gradient of
+    the direction plus noise."""
+    u = shade_uniforms(width, height)
+    zf, zn = u["zFar"], u["zNear"]
+    aspect = width / height
+    th = math.tan(math.radians(fov_deg) / 2.0)
+    yaw, pitch = math.radians(yaw_deg), math.radians(pitch_deg)
+    fwd = np.array([math.cos(pitch) * math.sin(yaw), math.sin(pitch), -math.cos(pitch) * math.cos(yaw)])
+    right = np.cross(fwd, [0.0, 1.0, 0.0])
+    right /= np.linalg.norm(right)
+    up = np.cross(right, fwd)
+    eye = np.array([0.0, eye_height, 0.0])
+    view = np.eye(4)
+    view[0, :3], view[1, :3], view[2, :3] = right, up, -fwd
+    view[:3, 3] = -view[:3, :3] @ eye
+    proj = np.zeros((4, 4))
+    proj[0, 0], proj[1, 1] = 1.0 / (aspect * th), 1.0 / th
+    proj[2, 2], proj[2, 3], proj[3, 2] = -(zf + zn) / (zf - zn), -2.0 * zf * zn / (zf - zn), -1.0
+    vp = proj @ view
+
+    def window_depth(dist_along_fwd):
+        z_ndc = (zf + zn) / (zf - zn) - 2.0 * zf * zn / ((zf - zn) * dist_along_fwd)
+        return 0.5 * z_ndc + 0.5
+
+    rng = SplitMix64(seed)
+    ks = [(rng.uniform(0.2, 1.5), rng.uniform(0.0, 2.0 * math.pi), rng.uniform(0.0, 2.0 * math.pi),
+           rng.uniform(0.02, 0.08)) for _ in range(6)]
+    frags = np.zeros((height, width, 12), dtype=np.float32)
+    cover = np.zeros((height, width), dtype=np.uint8)
+    depth = np.ones((height, width), dtype=np.float32)
+    xs = ((np.arange(width) + 0.5) / width) * 2.0 - 1.0
+    shorten = uniform_array(seed * 31 + 5, width * height).reshape(height, width)
+    for row in range(height):
+        y = (row + 0.5) / height * 2.0 - 1.0
+        d = fwd[None, :] + (xs[:, None] * th * aspect) * right[None, :] + (y * th) * up[None, :]
+        down = d[:, 1] < -1e-6
+        t = np.where(down, eye_height / np.where(down, -d[:, 1], 1.0), np.inf)
+        P = eye[None, :] + t[:, None] * d
+        along = t * (d @ fwd)
+        ok = down & (along < zf * 0.999) & (along > zn)
+        px, pz = np.where(ok, P[:, 0], 0.0), np.where(ok, P[:, 2], 0.0)
+        hx = np.zeros(width)
+        hz = np.zeros(width)
+        hh = np.zeros(width)
+        for k, ang, ph, amp in ks:
+            arg = k * (math.cos(ang) * px + math.sin(ang) * pz) + ph
+            hh += amp * np.sin(arg)
+            hx += amp * k * math.cos(ang) * np.cos(arg)
+            hz += amp * k * math.sin(ang) * np.cos(arg)
+        n = np.stack([-hx, np.ones(width), -hz], axis=1)
+        n /= np.linalg.norm(n, axis=1)[:, None]
+        n *= (1.0 - 0.02 * shorten[row])[:, None]
+        Pw = np.stack([px, hh, pz], axis=1)
+        vv = eye[None, :] - Pw
+        cy, sy = math.cos(yaw), math.sin(yaw)
+        ny = np.stack([cy * n[:, 0] - sy * n[:, 2], n[:, 1], sy * n[:, 0] + cy * n[:, 2]], axis=1)
+        ny /= np.linalg.norm(ny, axis=1)[:, None]
+        h0 = np.stack([px, np.zeros(width), pz, np.ones(width)], axis=1) @ vp.T
+        ndc0 = h0[:, :2] / np.where(ok, h0[:, 3], 1.0)[:, None]
+        fz = window_depth(np.where(ok, along, 1.0))
+        rec = np.concatenate([n, vv, ny, ndc0, fz[:, None]], axis=1)
+        frags[row] = np.where(ok[:, None], rec, 0.0).astype(np.float32)
+        cover[row] = ok
+        bed = 1.0 + 5.0 * (0.5 + 0.5 * np.sin(0.05 * px + 0.3) * np.cos(0.07 * pz))
+        tb = np.where(down, (eye_height + bed) / np.where(down, -d[:, 1], 1.0), np.inf)
+        bed_along = tb * (d @ fwd)
+        hit = down & (bed_along < zf)
+        depth[row] = np.where(hit, window_depth(np.where(hit, bed_along, 1.0)), 1.0).astype(np.float32)
+    lw, lh = max(1, int(round(width * local_scale))), max(1, int(round(height * local_scale)))
+    refl = _noise_rgba(seed * 3 + 1, lh, lw, "mask")
+    refr = _noise_rgba(seed * 3 + 2, lh, lw, 255)
+    sky = np.empty((6, sky_length, sky_length, 4), dtype=np.uint8)
+    for f in range(6):
+        dirs = cube_texel_directions(sky_length, face=f)
+        dirs = dirs / np.linalg.norm(dirs, axis=-1, keepdims=True)
+        grain = uniform_array(seed * 3 + 3 + 1000 * f, sky_length * sky_length).reshape(sky_length, sky_length)
+        sky[f, ..., 0] = np.clip(np.rint((0.35 + 0.3 * dirs[..., 0] + 0.02 * grain) * 255), 0, 255)
+        sky[f, ..., 1] = np.clip(np.rint((0.55 + 0.3 * dirs[..., 1] + 0.02 * grain) * 255), 0, 255)
+        sky[f, ..., 2] = np.clip(np.rint((0.75 + 0.2 * dirs[..., 2] + 0.02 * grain) * 255), 0, 255)
+    sky[..., 3] = 255
+    return dict(uniforms=u, fragments=frags, coverage=cover, depth=depth, reflections=refl, refractions=refr,
+                skybox=sky, view_projection=vp)
