@@ -130,14 +130,27 @@ __device__ __forceinline__ void step_snapshot(int s, float& c, float& sn, float&
   sn = snn;
 }
 
-// Forward of one wave at the thread's points: add its displacement to every snapshot.
-template <int SMAX, int KP, bool kCheb>
-__device__ __forceinline__ void forward_wave(const FitWaveLds& wv, int S, const double (&x)[KP],
-                                             const double (&z)[KP], float (&r)[KP][SMAX][3]) {
+// (cos, sin) of a wave's phase at the thread's points at t0: FP64 reduction to
+// revolutions, then the hardware sine and cosine (which take revolutions).
+template <int KP>
+__device__ __forceinline__ void wave_seeds(const FitWaveLds& wv, const double (&x)[KP], const double (&z)[KP],
+                                           float (&c)[KP], float (&sn)[KP]) {
 #pragma unroll
   for (int k = 0; k < KP; ++k) {
     const float ph = (float)__builtin_amdgcn_fract(fma(wv.kx, x[k], fma(wv.kz, z[k], wv.c0)));
-    float c = __builtin_amdgcn_cosf(ph), sn = __builtin_amdgcn_sinf(ph), cp = 0.0f, sp = 0.0f;
+    c[k] = __builtin_amdgcn_cosf(ph);
+    sn[k] = __builtin_amdgcn_sinf(ph);
+  }
+}
+
+// Forward of one wave at the thread's points, from its seeds: add its displacement to
+// every snapshot.
+template <int SMAX, int KP, bool kCheb>
+__device__ __forceinline__ void forward_wave(const FitWaveLds& wv, const float (&c0)[KP], const float (&s0)[KP],
+                                             float (&r)[KP][SMAX][3]) {
+#pragma unroll
+  for (int k = 0; k < KP; ++k) {
+    float c = c0[k], sn = s0[k], cp = 0.0f, sp = 0.0f;
 #pragma unroll
     for (int s = 0; s < SMAX; ++s) {
       r[k][s][0] = fmaf(wv.qadx, c, r[k][s][0]);
@@ -152,9 +165,9 @@ __device__ __forceinline__ void forward_wave(const FitWaveLds& wv, int S, const 
 // per point, then the five derivatives, added over the thread's points, reduced over
 // the wavefront and written to the wave's rows of s_red.
 template <int SMAX, int KP, bool kCheb>
-__device__ __forceinline__ void backward_wave(const FitDesc& P, const FitWaveLds& wv, int S, const double (&x)[KP],
-                                              const double (&z)[KP], const float (&r)[KP][SMAX][3], float* s_red,
-                                              int lane, int wid) {
+__device__ __forceinline__ void backward_wave(const FitDesc& P, const FitWaveLds& wv, const float (&c0)[KP],
+                                              const float (&s0)[KP], const double (&x)[KP], const double (&z)[KP],
+                                              const float (&r)[KP][SMAX][3], float* s_red, int lane, int wid) {
   const float x0f = (float)P.x0, z0f = (float)P.z0;
   const float dtf = (float)P.dt, Sf = (float)SMAX, t0f = (float)P.t0;
   const float A = wv.A, Q = wv.Q, Dx = wv.Dx, Dz = wv.Dz;
@@ -162,8 +175,7 @@ __device__ __forceinline__ void backward_wave(const FitDesc& P, const FitWaveLds
   float gA = 0.0f, gL = 0.0f, gAl = 0.0f, gQ = 0.0f, gP = 0.0f;
 #pragma unroll
   for (int k = 0; k < KP; ++k) {
-    const float ph = (float)__builtin_amdgcn_fract(fma(wv.kx, x[k], fma(wv.kz, z[k], wv.c0)));
-    float c = __builtin_amdgcn_cosf(ph), sn = __builtin_amdgcn_sinf(ph), cp = 0.0f, sp = 0.0f;
+    float c = c0[k], sn = s0[k], cp = 0.0f, sp = 0.0f;
     float sxc = 0, szc = 0, syc = 0, sxs = 0, szs = 0, sys = 0, vyc = 0, vxs = 0, vzs = 0;
 #pragma unroll
     for (int s = 0; s < SMAX; ++s) {
@@ -215,6 +227,48 @@ __device__ __forceinline__ void backward_wave(const FitDesc& P, const FitWaveLds
   }
 }
 
+// Forward passes of waves [i0, i1). Each iteration seeds the next wave first: its FP64
+// phase reduction, sine and cosine (long-latency, and independent of the current wave)
+// then overlap the current wave's snapshot steps instead of stalling at the top of the
+// next iteration.
+template <int SMAX, int KP, bool kCheb>
+__device__ __forceinline__ void forward_waves(const FitWaveLds* s_wave, int i0, int i1, const double (&x)[KP],
+                                              const double (&z)[KP], float (&r)[KP][SMAX][3]) {
+  if (i0 >= i1) return;
+  float c[KP], sn[KP];
+  wave_seeds<KP>(s_wave[i0], x, z, c, sn);
+  for (int i = i0; i < i1; ++i) {
+    float cn[KP], snn[KP];
+    wave_seeds<KP>(s_wave[i + 1 < i1 ? i + 1 : i], x, z, cn, snn);
+    forward_wave<SMAX, KP, kCheb>(s_wave[i], c, sn, r);
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+      c[k] = cn[k];
+      sn[k] = snn[k];
+    }
+  }
+}
+
+// Backward passes of waves [i0, i1), seeded one wave ahead as in forward_waves.
+template <int SMAX, int KP, bool kCheb>
+__device__ __forceinline__ void backward_waves(const FitDesc& P, const FitWaveLds* s_wave, int i0, int i1,
+                                               const double (&x)[KP], const double (&z)[KP],
+                                               const float (&r)[KP][SMAX][3], float* s_red, int lane, int wid) {
+  if (i0 >= i1) return;
+  float c[KP], sn[KP];
+  wave_seeds<KP>(s_wave[i0], x, z, c, sn);
+  for (int i = i0; i < i1; ++i) {
+    float cn[KP], snn[KP];
+    wave_seeds<KP>(s_wave[i + 1 < i1 ? i + 1 : i], x, z, cn, snn);
+    backward_wave<SMAX, KP, kCheb>(P, s_wave[i], c, sn, x, z, r, s_red, lane, wid);
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+      c[k] = cn[k];
+      sn[k] = snn[k];
+    }
+  }
+}
+
 // s_wave holds the problem's waves with the ncheb three-term-recurrence waves first, so
 // each loop over waves is branch-free.
 template <int SMAX, int KP>
@@ -242,9 +296,8 @@ __device__ __forceinline__ void adjoint_body(const FitDesc& P, int local_block, 
     for (int s = 0; s < SMAX; ++s) r[k][s][0] = r[k][s][1] = r[k][s][2] = 0.0f;
 
   // forward: d per snapshot
-  int i = 0;
-  for (; i < ncheb; ++i) forward_wave<SMAX, KP, true>(s_wave[i], S, x, z, r);
-  for (; i < P.W; ++i) forward_wave<SMAX, KP, false>(s_wave[i], S, x, z, r);
+  forward_waves<SMAX, KP, true>(s_wave, 0, ncheb, x, z, r);
+  forward_waves<SMAX, KP, false>(s_wave, ncheb, P.W, x, z, r);
   // residuals and loss
   float loss = 0.0f;
 #pragma unroll
@@ -252,13 +305,15 @@ __device__ __forceinline__ void adjoint_body(const FitDesc& P, int local_block, 
 #pragma unroll
     for (int s = 0; s < SMAX; ++s) {
       // snapshots past S (and points past the grid) get zero residuals; their loads
-      // read the last snapshot so every address stays inside the target
-      const bool use = active[k] && s < S;
+      // read the last snapshot so every address stays inside the target. The mask is a
+      // factor rather than a select: a select on a loaded value is turned into a branch
+      // around the load, which then waits for memory once per snapshot.
+      const float use = active[k] && s < S ? 1.0f : 0.0f;
       const float* tg = P.target + ((size_t)(s < S ? s : S - 1) * V + vv[k]) * 3;
       const float t0 = tg[0], t1 = tg[1], t2 = tg[2];
-      r[k][s][0] = use ? r[k][s][0] - t0 : 0.0f;
-      r[k][s][1] = use ? r[k][s][1] - t1 : 0.0f;
-      r[k][s][2] = use ? r[k][s][2] - t2 : 0.0f;
+      r[k][s][0] = (r[k][s][0] - t0) * use;
+      r[k][s][1] = (r[k][s][1] - t1) * use;
+      r[k][s][2] = (r[k][s][2] - t2) * use;
       loss = fmaf(0.5f * r[k][s][0], r[k][s][0], loss);
       loss = fmaf(0.5f * r[k][s][1], r[k][s][1], loss);
       loss = fmaf(0.5f * r[k][s][2], r[k][s][2], loss);
@@ -268,8 +323,8 @@ __device__ __forceinline__ void adjoint_body(const FitDesc& P, int local_block, 
   if (lane == 63) s_red[(P.W * 5) * 4 + wid] = lsum;
 
   // backward
-  for (i = 0; i < ncheb; ++i) backward_wave<SMAX, KP, true>(P, s_wave[i], S, x, z, r, s_red, lane, wid);
-  for (; i < P.W; ++i) backward_wave<SMAX, KP, false>(P, s_wave[i], S, x, z, r, s_red, lane, wid);
+  backward_waves<SMAX, KP, true>(P, s_wave, 0, ncheb, x, z, r, s_red, lane, wid);
+  backward_waves<SMAX, KP, false>(P, s_wave, ncheb, P.W, x, z, r, s_red, lane, wid);
 }
 
 // Stage the problem's waves into LDS, three-term-recurrence waves first (each group in
