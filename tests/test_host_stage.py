@@ -67,6 +67,27 @@ def test_host_stage_default_scene(tmp_path):
         print(t, res, flush=True)
         assert res["pos_max"] <= POS_TOL and res["nrm_max"] <= NRM_TOL
         assert res["tan_max"] <= NRM_TOL and res["bit_max"] <= NRM_TOL
+    # WaterStage::shade on a 4 x 2 frame (default settings at 9 h) against the oracle
+    import oracle as orc
+    from wavetool_amd import scenes
+    sh = (tmp_path / "stage.bin.shade").read_bytes()
+    P, o = 8, 0
+    frags = np.frombuffer(sh, dtype=np.float32, count=P * 12, offset=o).reshape(2, 4, 12)
+    o += P * 48
+    depth = np.frombuffer(sh, dtype=np.float32, count=P, offset=o).reshape(2, 4)
+    o += P * 4
+    refl = np.frombuffer(sh, dtype=np.uint8, count=P * 4, offset=o).reshape(2, 4, 4)
+    o += P * 4
+    refr = np.frombuffer(sh, dtype=np.uint8, count=P * 4, offset=o).reshape(2, 4, 4)
+    o += P * 4
+    sky = np.frombuffer(sh, dtype=np.uint8, count=6 * 2 * 2 * 4, offset=o).reshape(6, 2, 2, 4)
+    o += 96
+    col = np.frombuffer(sh, dtype=np.float32, count=P * 4, offset=o).reshape(2, 4, 4)
+    sd = orc.ShadeData(scenes.shade_uniforms(4, 2), depth, refl, refr, sky)
+    ref = orc.water_shade(sd, frags, None)
+    # near fragments and smooth inputs: FP32 rounding only (the fragment-stage tests hold
+    # the kernel to a per-pixel bound; here the host stage's uniforms and call are checked)
+    assert np.abs(col - ref).max() < 1e-4, np.abs(col - ref).max()
 
 
 def stb_like_red(img):
@@ -127,3 +148,39 @@ def test_heightmap_png_loader_rejects_bad_files(tmp_path):
     trunc.write_bytes(trunc.read_bytes()[:40])
     r = subprocess.run([CHECK, "--png", str(trunc), str(tmp_path / "o.bin")], capture_output=True, text=True)
     assert r.returncode != 0
+
+
+def _shade_uniforms(hours, w, h):
+    out = subprocess.run([CHECK, "--shade-uniforms", str(hours), str(w), str(h)], capture_output=True, text=True,
+                         check=True).stdout.split("\n")
+    fog, sun, rest = ([float(v) for v in line.split()] for line in out[:3])
+    return fog, sun, rest
+
+
+@pytest.mark.parametrize("hours,sun,dim", [(0.0, (0.0, -1.0), 0.0), (6.0, (1.0, 0.0), 0.0), (12.0, (0.0, 1.0), 1.0),
+                                           (18.0, (-1.0, 0.0), 0.0)])
+def test_shade_uniforms_sun_and_fog(hours, sun, dim):
+    """RenderEngine::render's sun (src/render-engine.cpp:289-291) runs round the unit
+    circle from straight down at midnight, through the horizon at 6 h, to straight up
+    at noon; the far fog colour (:295) is the noon colour scaled by the sun's height
+    clamped to [0, 1]. The rest are the reference's defaults (render-engine.h:213-235)."""
+    if not os.path.exists(CHECK):
+        pytest.fail("water_stage_check is not built (run __graft_entry__.build())")
+    fog, s, rest = _shade_uniforms(hours, 800, 600)
+    assert s[0] == pytest.approx(sun[0], abs=1e-6) and s[1] == pytest.approx(sun[1], abs=1e-6) and s[2] == 0.0
+    assert fog[:3] == pytest.approx([max(0.0, min(1.0, s[1]))] * 3, abs=1e-7)
+    assert fog[:3] == pytest.approx([dim] * 3, abs=1e-6) and fog[3] == pytest.approx(0.1)
+    assert rest[:11] == pytest.approx([1.0, 0.0, 0.05, 50.0, 1.0, 0.1, 800.0, 600.0, 0.3, 100.0, 0.1])
+
+
+def test_shade_uniforms_match_the_python_scene_defaults():
+    """The host stage and the generator the fragment tests and bench use form the same
+    uniforms (9 h, the reference's default time of day)."""
+    if not os.path.exists(CHECK):
+        pytest.fail("water_stage_check is not built (run __graft_entry__.build())")
+    from wavetool_amd import scenes
+    fog, sun, rest = _shade_uniforms(9.0, 1024, 768)
+    u = scenes.shade_uniforms(1024, 768)
+    assert fog == pytest.approx(u["fogColourFarAtCurrentTime"], abs=1e-7)
+    assert sun == pytest.approx(u["sunPosition"], abs=1e-7)
+    assert rest[6:8] == [1024.0, 768.0]

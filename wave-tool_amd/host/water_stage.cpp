@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -191,6 +192,42 @@ wt_status WaterStage::varyings(const wt_camera& camera, double time, const Water
   p.waveAnimationTimeInSeconds = time;
   const wt_status st = wt_vertex_varyings(ctx_, &p, heightmap, vp, vy, grid_.positions(), grid_.normals(),
                                           grid_.vertexCount(), &out, stream);
+  return st == WT_OK ? WT_OK : fail(st, wt_last_error(ctx_));
+}
+
+wt_shade_params shadeUniforms(const ShadeSettings& s, uint32_t windowWidth, uint32_t windowHeight) {
+  wt_shade_params u;
+  std::memset(&u, 0, sizeof u);
+  // sun position on the unit circle in the x-y plane, at -90 degrees at midnight
+  const float timeOfDayInDays = s.timeOfDayInHours / 24.0f;
+  const float theta = timeOfDayInDays * 6.28318530717958647692f - 1.57079632679489661923f;
+  u.sunPosition[0] = std::cos(theta);
+  u.sunPosition[1] = std::sin(theta);
+  u.sunPosition[2] = 0.0f;
+  const float dim = std::min(std::max(u.sunPosition[1], 0.0f), 1.0f);
+  for (int k = 0; k < 3; ++k) u.fogColourFarAtCurrentTime[k] = dim * s.fogColourFarAtNoon[k];
+  u.fogColourFarAtCurrentTime[3] = s.fogColourFarAtNoon[3];
+  u.fogDepthRadiusFar = s.fogDepthRadiusFar;
+  u.fogDepthRadiusNear = s.fogDepthRadiusNear;
+  u.softEdgesDeltaDepthThreshold = s.softEdgesDeltaDepthThreshold;
+  u.sunShininess = s.sunShininess;
+  u.sunStrength = s.sunStrength;
+  u.tintDeltaDepthThreshold = s.tintDeltaDepthThreshold;
+  u.viewportWidthHeight[0] = (float)windowWidth;
+  u.viewportWidthHeight[1] = (float)windowHeight;
+  u.waterClarity = s.waterClarity;
+  u.zFar = s.zFar;
+  u.zNear = s.zNear;
+  return u;
+}
+
+wt_status WaterStage::shade(const ShadeSettings& settings, const wt_shade_textures& textures,
+                            const wt_water_fragment* fragments, const uint8_t* coverage, uint32_t width,
+                            uint32_t height, float* colour, bool blend, void* stream) {
+  if (!ctx_) return fail(WT_ERR_INVALID_ARG, "WaterStage::shade before create");
+  const wt_shade_params u = shadeUniforms(settings, width, height);
+  const wt_status st =
+      wt_water_shade(ctx_, &u, &textures, fragments, coverage, width, height, colour, blend ? 1 : 0, stream);
   return st == WT_OK ? WT_OK : fail(st, wt_last_error(ctx_));
 }
 

@@ -93,6 +93,28 @@ struct WaterSettings {
   float heightmapScrollVelocity[2] = {0.0f, 0.0f};
 };
 
+// RenderEngine's scene parameters the water fragment shader reads
+// (src/render-engine.h:213-235, the defaults here are the reference's).
+struct ShadeSettings {
+  float timeOfDayInHours = 9.0f;                           // in [0, 24]
+  float fogColourFarAtNoon[4] = {1.0f, 1.0f, 1.0f, 0.1f};  // rgb tint, a = density
+  float fogDepthRadiusFar = 1.0f;
+  float fogDepthRadiusNear = 0.0f;
+  float softEdgesDeltaDepthThreshold = 0.05f;
+  float sunShininess = 50.0f;
+  float sunStrength = 1.0f;
+  float tintDeltaDepthThreshold = 0.1f;
+  float waterClarity = 0.3f;
+  float zFar = 100.0f;  // Z_FAR
+  float zNear = 0.1f;   // Z_NEAR
+};
+
+// The fragment uniforms RenderEngine::render forms each frame: the sun direction from
+// the time of day (src/render-engine.cpp:289-291), the fog colour dimmed by the sun's
+// height (:295), the viewport from the window size (:1093), the rest copied
+// (:1060-1098). FP32 throughout, as the reference computes them.
+wt_shade_params shadeUniforms(const ShadeSettings& s, uint32_t windowWidth, uint32_t windowHeight);
+
 // The per-frame water update of RenderEngine::render (src/render-engine.cpp:796-1115)
 // with the draw replaced by the displacement kernel writing the grid's buffers.
 class WaterStage {
@@ -120,6 +142,13 @@ class WaterStage {
   // skips it). Call after update() on the same stream.
   wt_status varyings(const wt_camera& camera, double waveAnimationTimeInSeconds, const WaterSettings& settings,
                      const wt_heightmap* heightmap, const wt_varying_buffers& out, void* stream);
+  // The fragment half of the water draw (water-grid.frag:70-162) as a deferred pass:
+  // shadeUniforms(settings, width, height), then wt_water_shade over a fragment buffer
+  // the renderer rasterised, blended over `colour` when `blend` (the reference's blend
+  // state, src/render-engine.cpp:302-303).
+  wt_status shade(const ShadeSettings& settings, const wt_shade_textures& textures,
+                  const wt_water_fragment* fragments, const uint8_t* coverage, uint32_t width, uint32_t height,
+                  float* colour, bool blend, void* stream);
   const WaterGrid& grid() const { return grid_; }
   const double* corners() const { return &corners_[0][0]; }
   const std::string& lastError() const { return err_; }
