@@ -68,6 +68,31 @@ __device__ __forceinline__ float wave_sum63(float v) {
   return v;
 }
 
+// The sum over each row of 16 lanes, in every lane of the row (the first four steps of
+// wave_sum63).
+__device__ __forceinline__ float row_sum(float v) {
+  v += dpp_term<0xB1, 0xf>(v);
+  v += dpp_term<0x4E, 0xf>(v);
+  v += dpp_term<0x141, 0xf>(v);
+  v += dpp_term<0x140, 0xf>(v);
+  return v;
+}
+
+// gfx950's lane swaps between two registers, summed: with x, y holding row sums (rows
+// r0..r3 of 16 lanes),
+//   swap32_sum(x, y) = [x.r0 + x.r2, x.r1 + x.r3, y.r0 + y.r2, y.r1 + y.r3],
+//   swap16_sum(x, y) = [x.r0 + x.r1, y.r0 + y.r1, x.r2 + x.r3, y.r2 + y.r3].
+// Two values share each step, so reducing several values across the wavefront takes
+// about half the instructions of one broadcast chain per value.
+__device__ __forceinline__ float swap32_sum(float x, float y) {
+  const auto p = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+  return __uint_as_float(p[0]) + __uint_as_float(p[1]);
+}
+__device__ __forceinline__ float swap16_sum(float x, float y) {
+  const auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+  return __uint_as_float(p[0]) + __uint_as_float(p[1]);
+}
+
 // The snapshot step omega dt of wave i in radians, reduced to [0, 2 pi).
 __device__ __forceinline__ double fit_wave_step(const FitDesc& P, int i) {
   const float* pr = P.params + 5 * i;
@@ -163,7 +188,8 @@ __device__ __forceinline__ void forward_wave(const FitWaveLds& wv, const float (
 
 // Backward of one wave: six sums over snapshots and three sums of their running values
 // per point, then the five derivatives, added over the thread's points, reduced over
-// the wavefront and written to the wave's rows of s_red.
+// the wavefront (row sums, then lane swaps that leave dA, dalpha, dL, dQ in rows 0..3
+// and dphi0 in row 0) and written to the wave's rows of s_red.
 template <int SMAX, int KP, bool kCheb>
 __device__ __forceinline__ void backward_wave(const FitDesc& P, const FitWaveLds& wv, const float (&c0)[KP],
                                               const float (&s0)[KP], const double (&x)[KP], const double (&z)[KP],
@@ -212,18 +238,14 @@ __device__ __forceinline__ void backward_wave(const FitDesc& P, const FitWaveLds
     gQ += A * sec;
     gP += g0;
   }
-  gA = wave_sum63(gA);
-  gL = wave_sum63(gL);
-  gAl = wave_sum63(gAl);
-  gQ = wave_sum63(gQ);
-  gP = wave_sum63(gP);
-  if (lane == 63) {
+  const float t = swap16_sum(swap32_sum(row_sum(gA), row_sum(gL)), swap32_sum(row_sum(gAl), row_sum(gQ)));
+  const float p = row_sum(gP);
+  const float tp = swap16_sum(swap32_sum(p, p), swap32_sum(p, p));
+  if ((lane & 15) == 15) {
+    const int row = lane >> 4;
     float* o = s_red + (wv.orig * 5) * 4 + wid;
-    o[0 * 4] = gA;
-    o[1 * 4] = gL;
-    o[2 * 4] = gAl;
-    o[3 * 4] = gQ;
-    o[4 * 4] = gP;
+    o[(((row & 1) << 1) | (row >> 1)) * 4] = t;  // rows 0..3 hold dA, dalpha, dL, dQ
+    if (row == 0) o[4 * 4] = tp;
   }
 }
 
