@@ -18,8 +18,9 @@
 //     running sums over snapshots plus three running sums of those sums (the
 //     time-weighted sums follow by summation by parts: sum_s s f_s = S U - V), from
 //     which the five parameter derivatives of each grid point follow; the thread's KP
-//     points are added, then a fixed-order DPP wavefront reduction, a fixed-order sum
-//     over the 4 wavefronts, and one partial per (parameter, block) in global memory.
+//     points are added, then a fixed-order wavefront reduction (DPP row sums, lane
+//     swaps across rows), a fixed-order sum over the 4 wavefronts, and one partial per
+//     (parameter, block) in global memory.
 // reduce_adam_kernel: one workgroup per (problem, parameter row): fixed-order sum of
 //   the block partials, scaling to the mean misfit, and either an Adam step on the
 //   parameter or a plain gradient / loss write-out.
