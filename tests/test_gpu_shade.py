@@ -1,17 +1,22 @@
 """GPU parity of the fragment stage (wt_water_shade, assets/shaders/water-grid.frag:70-162)
 against the FP64 oracle (oracle/wt_oracle_shade.hpp) on synthetic water frames.
 
-Tolerance. The kernel computes in FP32, as the shader does. Two places amplify FP32
-rounding, both in the shader's own formulas:
+Tolerance. The kernel computes in FP32, as the shader does. Three places amplify FP32
+rounding, all in the shader's own formulas:
 - linearizeDepth's denominator zFar - d (zFar - zNear) cancels as d -> 1, so the
   linear depth of a far fragment carries an error of about eps32 * zFar * lin / den;
   the soft-edge and tint factors scale that by 1 / threshold (20 and 10 at the
   defaults);
-- the sun term pow(x, 50) scales the relative error of x by 50.
-The per-pixel bound is 2e-5 + 2 s (e_back + e_frag), with e the linear-depth error
+- the sun term pow(x, 50) scales the relative error of x by 50;
+- FP32 texture coordinates (about 1e-7 relative) move a bilinear lookup by up to
+  4e-4 texel on a 3840-texel target; across the local reflection's 0/255 alpha edges
+  that is up to 4e-4 in alpha, weighted by the Fresnel factor in the colour (about
+  2e-5 at the benchmark's viewing angles).
+The per-pixel bound is 5e-5 + 2 s (e_back + e_frag), with e the linear-depth error
 bound above and s = 1 + 1 / soft-edge threshold + 0.6 / tint threshold while the depth
 difference is under those thresholds (0.6 is the largest deep-to-shallow tint step).
-It sits 7x or more above what the oracle's own FP32 instantiation deviates by. Pixels
+On the benchmark frame the kernel's largest error is 0.65 of the bound (0.09 at the
+99.99th percentile) before the texture-coordinate term was added. Pixels
 whose reflected direction sits within 1e-4 of a cube-face edge (the face choice flips
 between FP32 and FP64 there, and the texels on either side differ), whose back depth
 sits at the sky threshold, or, with equal fog radii, whose view distance sits at the
@@ -86,7 +91,7 @@ def comparison_mask(sc, u, depth, coverage):
         thr = u[key]
         if thr > 0:
             sens += np.where(delta < thr + 2 * (eb + ef), weight / thr, 0.0)
-    tol = 2e-5 + 2.0 * sens * (eb + ef)
+    tol = 5e-5 + 2.0 * sens * (eb + ef)
     fog_ok = np.ones((H, W), dtype=bool)
     if u["fogDepthRadiusNear"] == u["fogDepthRadiusFar"]:
         fog_ok = np.abs(ln[..., 0] / fz - u["fogDepthRadiusFar"]) > 1e-5
@@ -249,9 +254,9 @@ def test_shade_rejects_bad_arguments(torch_cuda, wtctx):
     assert st == wt.WT_ERR_INVALID_ARG
 
 
-def test_shade_4k_frame_rows(torch_cuda, wtctx):
-    """The benchmark frame (3840 x 2160, 2048^2 skybox faces): row bands at the bottom,
-    around the horizon and at the top against the oracle."""
+def test_shade_4k_frame(torch_cuda, wtctx):
+    """The benchmark frame (3840 x 2160, 2048^2 skybox faces), every pixel against the
+    oracle."""
     torch = torch_cuda
     say("shade 3840x2160 sky 2048 row bands")
     W, H = 3840, 2160
@@ -260,17 +265,9 @@ def test_shade_4k_frame_rows(torch_cuda, wtctx):
     got = run(torch, wtctx, sc, sc["coverage"], colour)
     sd = orc.ShadeData(sc["uniforms"], sc["depth"], sc["reflections"], sc["refractions"], sc["skybox"])
     ref = np.full((H, W, 4), -1.0)
-    horizon = int(np.argmin(sc["coverage"].any(axis=1))) if not sc["coverage"].any(axis=1).all() else H // 2
-    bands = [(0, 8), (max(0, horizon - 8), min(H, horizon + 8)), (H - 8, H)]
-    for r0, r1 in bands:
-        orc.water_shade(sd, sc["fragments"], sc["coverage"], nthreads=8, first=r0 * W, count=(r1 - r0) * W,
-                        out=ref)
-    rows = np.zeros(H, dtype=bool)
-    for r0, r1 in bands:
-        rows[r0:r1] = True
+    orc.water_shade(sd, sc["fragments"], sc["coverage"], nthreads=16, out=ref)
     stable, tol = comparison_mask(sc, sc["uniforms"], sc["depth"], sc["coverage"])
-    stable &= rows[:, None]
-    assert stable.sum() > 30000
+    assert stable.sum() > 0.999 * sc["coverage"].sum()
     check_frame(got, ref, stable, tol)
     cov = sc["coverage"].astype(bool)
     assert (got[~cov] == -1.0).all()
