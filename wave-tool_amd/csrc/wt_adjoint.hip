@@ -397,23 +397,32 @@ __global__ __launch_bounds__(kFitThreads) void reduce_adam_kernel(const FitDesc*
                                                                   const int* __restrict__ row_prob, FitAdam adam,
                                                                   int mode, float* __restrict__ grad_out,
                                                                   float* __restrict__ loss_out) {
-  __shared__ float s[kFitThreads];
+  static_assert(kFitThreads == 256, "the final sum reads four wavefront partials");
+  __shared__ float s[kFitThreads / 64];
   const int s_prob = load_uniform(row_prob, (long long)blockIdx.x);
   const FitDesc P = load_uniform(probs, s_prob);
   const int row = (int)(blockIdx.x - P.row_offset);
+  const int nparam = P.W * 5;
+  // the row's Adam state, loaded while the partials are summed (a kernel this short is
+  // bound by its chain of memory round trips)
+  float m_old = 0.0f, v_old = 0.0f, p_old = 0.0f;
+  int t = 0;
+  if (threadIdx.x == 0 && mode != 0 && row != nparam) {
+    m_old = P.adam_m[row];
+    v_old = P.adam_v[row];
+    p_old = P.params[row];
+    t = P.step[0];
+  }
   const float* src = P.partials + (size_t)row * P.nblocks;
   float acc = 0.0f;
   for (int b = threadIdx.x; b < P.nblocks; b += blockDim.x) acc += src[b];
-  s[threadIdx.x] = acc;
+  // fixed order: per wavefront by DPP, then the wavefronts in index order
+  acc = wave_sum63(acc);
+  if ((threadIdx.x & 63) == 63) s[threadIdx.x >> 6] = acc;
   __syncthreads();
-  for (int m = kFitThreads / 2; m >= 1; m >>= 1) {
-    if (threadIdx.x < m) s[threadIdx.x] += s[threadIdx.x + m];
-    __syncthreads();
-  }
   if (threadIdx.x != 0) return;
   const double inv = 1.0 / ((double)P.S * (double)P.n * (double)P.n);
-  const float total = (float)((double)s[0] * inv);
-  const int nparam = P.W * 5;
+  const float total = (float)((double)(((s[0] + s[1]) + s[2]) + s[3]) * inv);
   if (row == nparam) {  // loss row
     if (loss_out) loss_out[s_prob] = total;
     P.loss[0] = total;
@@ -424,15 +433,14 @@ __global__ __launch_bounds__(kFitThreads) void reduce_adam_kernel(const FitDesc*
     return;
   }
   // Adam (Kingma & Ba, 2015) with bias correction; step counter kept on the device
-  const int t = P.step[0];
   const float g = total;
-  float m = adam.beta1 * P.adam_m[row] + (1.0f - adam.beta1) * g;
-  float v2 = adam.beta2 * P.adam_v[row] + (1.0f - adam.beta2) * g * g;
+  float m = adam.beta1 * m_old + (1.0f - adam.beta1) * g;
+  float v2 = adam.beta2 * v_old + (1.0f - adam.beta2) * g * g;
   P.adam_m[row] = m;
   P.adam_v[row] = v2;
   const float bc1 = 1.0f - powf(adam.beta1, (float)t), bc2 = 1.0f - powf(adam.beta2, (float)t);
   const float mhat = m / bc1, vhat = v2 / bc2;
-  float p = P.params[row] - adam.lr * mhat / (sqrtf(vhat) + adam.eps);
+  float p = p_old - adam.lr * mhat / (sqrtf(vhat) + adam.eps);
   if (row % 5 == 1 && p < 1e-3f) p = 1e-3f;  // wavelength stays positive
   P.params[row] = p;
 }
