@@ -67,7 +67,7 @@ def load_traffic():
     each scaled by the same counter's reading for a plain copy kernel of known size.
     The write part (24 B per vertex) does not depend on the launch shape; the fetch part
     is the heightmap, once per XCD L2."""
-    path = os.path.join(ROOT, "profiles", "water_forward_config1_traffic.json")
+    path = os.path.join(ROOT, "profiles", "water_forward_1024sq_w64_traffic.json")
     if not os.path.exists(path):
         return None, None
     with open(path) as f:
@@ -89,10 +89,10 @@ def timed(torch, stream, fn, reps=1):
     return time.perf_counter() - t0, e0.elapsed_time(e1) * 1e-3
 
 
-# --------------------------------------------------------------------- config 1
+# --------------------------------------------------------------------- plain forward, 1024x1024 x 64 waves
 def bench_plain(torch, wt, scenes, args):
     dev = torch.device("cuda:0")
-    cfg = scenes.plain_config(1)
+    cfg = scenes.scene_1024_w64_heightmap()
     N, W, F = cfg["N"], cfg["W"], cfg["frames"]
     cam = cfg["camera"]
     cam_pos = cam[4:7]
@@ -140,7 +140,7 @@ def bench_plain(torch, wt, scenes, args):
     out = dict(metric=METRIC, value=vw_per_step * args.steps / wall, unit="vertex-waves/s", n_gpus=1,
                steps=args.steps, warmup=args.warmup, ms_per_step=wall * 1e3 / args.steps, higher_is_better=True,
                scaling="weak", vs_baseline=None, dtype="f32", data="synthetic",
-               config=dict(workload="config1: plain forward, 1024x1024 projected grid, 64 Gerstner waves, "
+               config=dict(workload="plain forward: 1024x1024 projected grid, 64 Gerstner waves, "
                                     "512x512 R32F value-noise heightmap, 240 frames per step (one HIP graph)",
                            grid=N, waves=W, frames_per_step=F, launch_shape=shape),
                roofline=roofline)
@@ -148,7 +148,7 @@ def bench_plain(torch, wt, scenes, args):
     return out, (corners, cam_pos, waves_np, texels, cfg)
 
 
-# --------------------------------------------------------------------- config 2
+# --------------------------------------------------------------------- batched forward, 512 patches
 def bench_batched_forward(torch, wt, scenes, frames=120):
     dev = torch.device("cuda:0")
     specs = scenes.batched_patches(seed=2, count=512)
@@ -203,7 +203,7 @@ def bench_batched_forward(torch, wt, scenes, frames=120):
     batch.close()
     ctx.close()
     batched = vw_frame * frames / wall
-    return dict(workload="config2: 512 patches (side 32-1024 log-uniform, 4-128 waves, own camera and t in "
+    return dict(workload="batched forward: 512 patches (side 32-1024 log-uniform, 4-128 waves, own camera and t in "
                          "[0, 3600) s), %d frames in one HIP graph; per frame the device projector re-aims "
                          "every patch, then one launch per segment shape displaces them" % frames,
                 value=batched, unit="vertex-waves/s", vertex_waves_per_frame=vw_frame,
@@ -220,10 +220,10 @@ def bench_batched_forward(torch, wt, scenes, frames=120):
 # --------------------------------------------------------------------- varyings
 def bench_varyings(torch, wt, scenes, launches=100):
     """The varyings pass (clip position, height-0 NDC, view vector, yaw-space normal,
-    heightmap sample) over the config-1 grid: an HBM-bound pass, 24 B read and 52 B
+    heightmap sample) over the 1024 x 1024 grid: an HBM-bound pass, 24 B read and 52 B
     written per vertex, timed as a captured graph of `launches` launches."""
     dev = torch.device("cuda:0")
-    cfg = scenes.plain_config(1)
+    cfg = scenes.scene_1024_w64_heightmap()
     N, W = cfg["N"], cfg["W"]
     V = N * N
     cam = cfg["camera"]
@@ -253,7 +253,7 @@ def bench_varyings(torch, wt, scenes, launches=100):
     ctx.close()
     t = ev / launches
     traffic = 76 * V
-    return dict(workload="varyings of the config-1 grid (1024 x 1024): clip position, height-0 NDC xy, view vector, "
+    return dict(workload="varyings of the 1024 x 1024 grid: clip position, height-0 NDC xy, view vector, "
                          "yaw-only view-space normal, heightmap sample",
                 value=V / t, unit="vertices/s", launch_us=round(t * 1e6, 3),
                 roofline=dict(bound="hbm", achieved=round(traffic / t / 1e9, 1), peak=8000.0, unit="GB/s",
@@ -310,7 +310,7 @@ def bench_shade(torch, wt, scenes, launches=50, cpu=True):
     return out
 
 
-# --------------------------------------------------------------------- configs 3 and 4
+# --------------------------------------------------------------------- wave fits, one and batched
 def make_fits(torch, wt, scenes, ctx, specs, sigma=0.01):
     """Device targets (synthesized from the hidden waves) and perturbed initial params."""
     dev = torch.device("cuda:0")
@@ -345,7 +345,7 @@ def bench_fit_single(torch, wt, scenes, iterations=500, cpu=True):
     graph.close()
     last = float(losses.item())
     vws = spec["n"] ** 2 * spec["S"] * spec["W"]
-    out = dict(workload="config3: one fit, 512x512 grid at 0.5 m, 16 snapshots, 32 waves, %d Adam iterations "
+    out = dict(workload="single fit: 512x512 grid at 0.5 m, 16 snapshots, 32 waves, %d Adam iterations "
                         "(one HIP graph)" % iterations,
                value=1.0 / wall, unit="fits/s", ms_per_iteration=wall * 1e3 / iterations, loss_last=last,
                adjoint_algorithmic_TFLOPs=round(wt._abi.WT_ADJOINT_FLOP_PER_VERTEX_WAVE_SNAPSHOT * vws * iterations
@@ -398,7 +398,7 @@ def bench_fit_batched(torch, wt, scenes, iterations=300, count=256):
         f.close()
     vws = sum(s["n"] ** 2 * s["S"] * s["W"] for s in specs)
     ctx.close()
-    return dict(workload="config4: %d fits (side 64-512 at 0.5 m, 8-64 waves, 4-32 snapshots), %d Adam iterations; "
+    return dict(workload="batched fits: %d fits (side 64-512 at 0.5 m, 8-64 waves, 4-32 snapshots), %d Adam iterations; "
                          "batched = one problem table, one HIP graph" % (count, iterations),
                 value=count / wall_b, unit="fits/s",
                 plain_one_by_one=dict(value=count / wall_p, unit="fits/s",
@@ -434,13 +434,13 @@ def main():
         print(json.dumps(bench_shade(torch, wt, scenes, cpu=not args.no_cpu_baseline)), flush=True)
         return
     out, inputs = bench_plain(torch, wt, scenes, args)
-    log("config1 %.4g vertex-waves/s, %.2f us per frame" % (out["value"], out["roofline"]["launch_us"]))
+    log("plain forward %.4g vertex-waves/s, %.2f us per frame" % (out["value"], out["roofline"]["launch_us"]))
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(*inputs)
     if not args.no_extras:
         extras = {}
         extras["batched_forward"] = bench_batched_forward(torch, wt, scenes)
-        log("config2 %.4g vertex-waves/s" % extras["batched_forward"]["value"])
+        log("batched forward %.4g vertex-waves/s" % extras["batched_forward"]["value"])
         extras["varyings"] = bench_varyings(torch, wt, scenes)
         log("varyings %.4g vertices/s, %.1f GB/s" % (extras["varyings"]["value"],
                                                       extras["varyings"]["roofline"]["achieved"]))
@@ -448,9 +448,9 @@ def main():
         log("fragment stage %.4g fragments/s, %.1f GB/s" % (extras["shade"]["value"],
                                                             extras["shade"]["roofline"]["achieved"]))
         extras["fit_single"] = bench_fit_single(torch, wt, scenes, cpu=not args.no_cpu_baseline)
-        log("config3 %.4g fits/s" % extras["fit_single"]["value"])
+        log("single fit %.4g fits/s" % extras["fit_single"]["value"])
         extras["fit_batched"] = bench_fit_batched(torch, wt, scenes)
-        log("config4 %.4g fits/s (%.2fx one by one)" % (extras["fit_batched"]["value"],
+        log("batched fits %.4g fits/s (%.2fx one by one)" % (extras["fit_batched"]["value"],
                                                          extras["fit_batched"]["speedup_vs_plain"]))
         out["extras"] = extras
     print(json.dumps(out), flush=True)

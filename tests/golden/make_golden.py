@@ -8,7 +8,7 @@ Fixtures (little-endian, numpy .npz without pickles, plus a JSON header):
                       src/camera.cpp:40-45, src/render-engine.h:214-237) at N = 17,
                       t in {0, 10, 3600}, analytic and finite-difference normals, with a
                       16x16 8-bit MIRRORED_REPEAT heightmap (s_h = 0.02, s_d = 1.0)
-  config0_n17.npz     the config-0 scene (4 seeded waves, t = 10 s) on a 17x17 grid, N/T/B
+  w4_t10_n17.npz      the 256x256 scene's waves (4 seeded waves, t = 10 s) on a 17x17 grid, N/T/B
   bilinear.npz        64 samples of a 16x16 R8 texture under REPEAT and MIRRORED_REPEAT
   adjoint_small.npz   loss and gradient of a 16x16 x 2 snapshots x 3 waves fit
 """
@@ -65,8 +65,8 @@ def make_default_scene():
                      displacement_scale=1.0, wrap="MIRRORED_REPEAT", format="R8_UNORM")
 
 
-def make_config0():
-    cfg = scenes.plain_config(0)
+def make_w4_t10():
+    cfg = scenes.scene_256_w4()
     st, corners, _ = orc.projector(cfg["camera"], cfg["amp"])
     waves = scenes.gen_waves(cfg["seed"], cfg["W"])
     sd = orc.SceneData(corners, cfg["camera"][4:7], 17, waves, cfg["t0"])
@@ -94,7 +94,7 @@ def make_adjoint():
         n=16, S=2, W=3, spacing=prob["spacing"], x0=prob["x0"], z0=prob["z0"], t0=prob["t0"], dt=prob["dt"])
 
 
-MAKERS = {"default_scene": make_default_scene, "config0_n17": make_config0, "bilinear": make_bilinear,
+MAKERS = {"default_scene": make_default_scene, "w4_t10_n17": make_w4_t10, "bilinear": make_bilinear,
           "adjoint_small": make_adjoint}
 
 

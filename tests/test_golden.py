@@ -27,7 +27,7 @@ def maker():
     return m
 
 
-@pytest.mark.parametrize("name", ["default_scene", "config0_n17", "bilinear", "adjoint_small"])
+@pytest.mark.parametrize("name", ["default_scene", "w4_t10_n17", "bilinear", "adjoint_small"])
 def test_oracle_reproduces_fixture(name):
     arrays, _ = maker().MAKERS[name]()
     ref = load(name)

@@ -1,6 +1,6 @@
 """GPU parity of the plain forward kernel (wt_water_displace) against the FP64 oracle.
 
-Tolerance (north-star contract): positions within 1e-4 m and normals within
+Tolerance: positions within 1e-4 m and normals within
 1e-4 rad for every vertex within 1 km of the camera. Normals of vertices whose
 heightmap sample lies within 2e-3 texel of a texel edge are left out of the normal
 check (the bilinear gradient is discontinuous there, so an FP32 and an FP64 sample
@@ -45,8 +45,7 @@ def gpu_displace(torch, ctx, params, waves_np, N, texels=None, fmt=wt.WT_TEXEL_R
     return out
 
 
-def scene_setup(cfg_index, frame=0):
-    cfg = scenes.plain_config(cfg_index)
+def scene_setup(cfg, frame=0):
     corners = wt.projector_corners(cfg["camera"], cfg["amp"])
     waves = scenes.gen_waves(cfg["seed"], cfg["W"])
     cam_pos = cfg["camera"][4:7]
@@ -73,10 +72,10 @@ def check_parity(torch, ctx, cfg, corners, waves, cam_pos, t, texels, with_tb, f
     return res
 
 
-def test_config0_parity(torch_cuda, wtctx):
+def test_grid256_w4_parity(torch_cuda, wtctx):
     """256^2 grid, 4 waves, t = 10 s, position + N/T/B against the FP64 oracle."""
-    say("config0 256x256 W=4 t=10 with T/B")
-    cfg, corners, waves, cam_pos, t, texels = scene_setup(0)
+    say("256x256 grid, W=4, t=10, with T/B")
+    cfg, corners, waves, cam_pos, t, texels = scene_setup(scenes.scene_256_w4())
     res = check_parity(torch_cuda, wtctx, cfg, corners, waves, cam_pos, t, texels, True)
     print(res, flush=True)
     assert res["n_near"] > 0.5 * cfg["N"] ** 2
@@ -86,10 +85,10 @@ def test_config0_parity(torch_cuda, wtctx):
 
 
 @pytest.mark.parametrize("frame", [0, 239])
-def test_config1_frame_parity(torch_cuda, wtctx, frame):
+def test_grid1024_w64_heightmap_frame_parity(torch_cuda, wtctx, frame):
     """1024^2 grid, 64 waves, 512^2 FP32 value-noise heightmap (REPEAT, scrolled)."""
-    say("config1 1024x1024 W=64 heightmap frame %d" % frame)
-    cfg, corners, waves, cam_pos, t, texels = scene_setup(1, frame)
+    say("1024x1024 grid, W=64, heightmap, frame %d" % frame)
+    cfg, corners, waves, cam_pos, t, texels = scene_setup(scenes.scene_1024_w64_heightmap(), frame)
     res = check_parity(torch_cuda, wtctx, cfg, corners, waves, cam_pos, t, texels, False)
     print(res, flush=True)
     assert res["pos_max"] <= POS_TOL
@@ -271,7 +270,7 @@ def test_graph_replay_matches_eager(torch_cuda, wtctx):
     synchronisation between replays; every replay equals the eager frames bit-exactly."""
     say("graph replay 12 frames")
     torch = torch_cuda
-    cfg, corners, waves, cam_pos, t, texels = scene_setup(1)
+    cfg, corners, waves, cam_pos, t, texels = scene_setup(scenes.scene_1024_w64_heightmap())
     N, F = 192, 12
     dev = torch.device("cuda")
     wv = torch.from_numpy(waves).to(dev)

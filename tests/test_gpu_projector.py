@@ -34,7 +34,7 @@ def camera_set():
         amps.append(float(rng.choice([0.0, 0.5, 2.0, 5.0])))
     special = [
         (cam(fov_deg=72.0, aspect=1.0, far=100.0, height=4.0, pitch_deg=0.0, z=70.0), 1.26),  # reference default
-        (cam(), 2.0),  # configs 0 and 1
+        (cam(), 2.0),  # the plain-grid scenes
         (cam(height=-3.0, pitch_deg=10.0), 2.0),  # underwater, looking up
         (cam(height=0.5, pitch_deg=-80.0), 2.0),  # inside the slab, looking down
         (cam(height=10.0, pitch_deg=0.0), 2.0),  # horizon: forward y clamped to -0.001

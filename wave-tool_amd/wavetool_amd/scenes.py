@@ -132,28 +132,31 @@ def camera_tuple(height=10.0, pitch_deg=-15.0, yaw_deg=-90.0, fov_deg=60.0, aspe
     return tuple(f(v) for v in (fov_deg, aspect, near, far, x, height, z, yaw_deg, pitch_deg))
 
 
-DISPLACEMENT_BOUND = 2.0  # projector slab half-height (BASELINE configs 0 and 1)
+DISPLACEMENT_BOUND = 2.0  # projector slab half-height of the plain-grid scenes
 
 
-def plain_config(index):
-    """Configs 0 (CPU, 256^2 x 4 waves, t = 10 s) and 1 (GPU, 1024^2 x 64 waves + 512^2
-    value-noise heightmap, 240 frames at t = 10 + k/60)."""
-    if index == 0:
-        return dict(N=256, W=4, seed=0, frames=1, t0=10.0, dt=0.0, heightmap=None, camera=camera_tuple(),
-                    sample_scale=0.0, displacement_scale=0.0, scroll=(0.0, 0.0), amp=DISPLACEMENT_BOUND,
-                    with_tb=True)
-    if index == 1:
-        # heightmap: 512^2 R32F value noise, x 0.05 m, scrolled 0.3 m/s along the wind
-        # (angle 0 -> +x), sampled with REPEAT at s_h = 0.02 (the reference default,
-        # render-engine.h:225): one texture tile spans 50 m.
-        return dict(N=1024, W=64, seed=1, frames=240, t0=10.0, dt=1.0 / 60.0, heightmap=dict(seed=1, size=512),
-                    camera=camera_tuple(), sample_scale=0.02, displacement_scale=0.05, scroll=(0.3, 0.0),
-                    amp=DISPLACEMENT_BOUND, with_tb=False)
-    raise ValueError(index)
+def scene_256_w4():
+    """256 x 256 projected grid, 4 seeded waves, one frame at t = 10 s, with tangents and
+    bitangents (the size a CPU finishes in seconds)."""
+    return dict(N=256, W=4, seed=0, frames=1, t0=10.0, dt=0.0, heightmap=None, camera=camera_tuple(),
+                sample_scale=0.0, displacement_scale=0.0, scroll=(0.0, 0.0), amp=DISPLACEMENT_BOUND,
+                with_tb=True)
+
+
+def scene_1024_w64_heightmap():
+    """1024 x 1024 projected grid, 64 seeded waves and a 512 x 512 value-noise heightmap,
+    240 frames at t = 10 + k/60 s.
+
+    Heightmap: R32F value noise x 0.05 m, scrolled 0.3 m/s along the wind (angle 0 -> +x),
+    sampled with REPEAT at s_h = 0.02 (the reference default, render-engine.h:225): one
+    texture tile spans 50 m."""
+    return dict(N=1024, W=64, seed=1, frames=240, t0=10.0, dt=1.0 / 60.0, heightmap=dict(seed=1, size=512),
+                camera=camera_tuple(), sample_scale=0.02, displacement_scale=0.05, scroll=(0.3, 0.0),
+                amp=DISPLACEMENT_BOUND, with_tb=False)
 
 
 def batched_patches(seed=2, count=512):
-    """Config 2: per patch grid side log-uniform in [32, 1024], W ~ U{4..128}, camera
+    """A batch of patches (512 by default): per patch grid side log-uniform in [32, 1024], W ~ U{4..128}, camera
     height U[2,50] m, pitch U[-35,-5] deg, yaw U[0,360) deg, t ~ U[0, 3600) s."""
     rng = SplitMix64(seed)
     out = []
@@ -170,7 +173,7 @@ def batched_patches(seed=2, count=512):
 
 
 def fit_problem(seed, n, S, W_hidden, W_fit, spacing=0.5, dt=0.25, noise=0.01):
-    """Configs 3/4: hidden waves (drawn as config 0, normalised by their count), target
+    """One fit problem: hidden waves (drawn as for the plain scenes, normalised by their count), target
     displacement on an n x n world grid (centred on the origin) at S snapshots
     t = s * dt, plus N(0, noise) on every component; a perturbed initial guess
     (every parameter x U[0.8, 1.2], direction + U[-15, 15] deg).
@@ -193,7 +196,7 @@ def fit_problem(seed, n, S, W_hidden, W_fit, spacing=0.5, dt=0.25, noise=0.01):
 
 
 def fit_batch_specs(seed=4, count=256):
-    """Config 4: per problem grid side log-uniform in [64, 512] (0.5 m spacing), hidden and
+    """A batch of fit problems (256 by default): per problem grid side log-uniform in [64, 512] (0.5 m spacing), hidden and
     fitted wave count U{8..64}, snapshots U{4..32} at 0.25 s, noise 0.01 m."""
     rng = SplitMix64(seed)
     out = []
