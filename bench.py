@@ -40,6 +40,20 @@ def cpu_threads():
     return min(16, os.cpu_count() or 1)
 
 
+def host_cpu():
+    """The host CPU model and its logical CPU count, for the CPU baselines' records."""
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return "%s, %d logical CPUs visible" % (model, os.cpu_count() or 1)
+
+
 def cpu_baseline(corners, cam_pos, waves, texels, cfg, max_frames=240, budget_s=10.0):
     """FP32 oracle instantiation, all host threads (up to 16), frames of the same
     workload (t = 10 + k/60) until about 10 s of wall time or the step's 240 frames."""
@@ -57,7 +71,7 @@ def cpu_baseline(corners, cam_pos, waves, texels, cfg, max_frames=240, budget_s=
         t_total += time.perf_counter() - t0
         frames += 1
     vw = N * N * W * frames
-    return dict(value=vw / t_total, unit="vertex-waves/s", cores=threads, kind="port",
+    return dict(value=vw / t_total, unit="vertex-waves/s", cores=threads, kind="port", host=host_cpu(),
                 sample="%d frame(s) of the same workload (%dx%d grid, %d waves, heightmap), FP32 oracle, "
                        "%d threads, %.2f s wall" % (frames, N, N, W, threads, t_total))
 
@@ -304,7 +318,7 @@ def bench_shade(torch, wt, scenes, launches=50, cpu=True):
             orc.water_shade(sd, sc["fragments"], sc["coverage"], precision="f32", nthreads=threads, out=buf)
             tc += time.perf_counter() - t0
             frames += 1
-        out["cpu_baseline"] = dict(value=covered * frames / tc, unit="fragments/s", cores=threads, kind="port",
+        out["cpu_baseline"] = dict(value=covered * frames / tc, unit="fragments/s", cores=threads, kind="port", host=host_cpu(),
                                    sample="%d whole frame(s), FP32 oracle, %d threads, %.2f s wall" % (frames, threads,
                                                                                                       tc))
     return out
@@ -359,7 +373,7 @@ def bench_fit_single(torch, wt, scenes, iterations=500, cpu=True):
         t0 = time.perf_counter()
         orc.fit_loss_grad(fd, prob["init"], nthreads=cpu_threads())
         it_s = time.perf_counter() - t0
-        out["cpu_baseline"] = dict(value=1.0 / (it_s * iterations), unit="fits/s", cores=cpu_threads(), kind="port",
+        out["cpu_baseline"] = dict(value=1.0 / (it_s * iterations), unit="fits/s", cores=cpu_threads(), kind="port", host=host_cpu(),
                                    sample="one FP64 oracle adjoint iteration (%.2f s wall), extrapolated to %d"
                                           % (it_s, iterations))
     fit.close()
