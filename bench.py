@@ -37,7 +37,17 @@ def log(msg):
 
 
 def cpu_threads():
-    return min(16, os.cpu_count() or 1)
+    """Host threads for the CPU baselines: every CPU this process may run on (its
+    affinity set), capped by OMP_NUM_THREADS where the host sets it to the share of
+    the CPUs that belongs to this job."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
 
 
 def host_cpu():
@@ -55,7 +65,7 @@ def host_cpu():
 
 
 def cpu_baseline(corners, cam_pos, waves, texels, cfg, max_frames=240, budget_s=10.0):
-    """FP32 oracle instantiation, all host threads (up to 16), frames of the same
+    """FP32 oracle instantiation on cpu_threads() host threads, frames of the same
     workload (t = 10 + k/60) until about 10 s of wall time or the step's 240 frames."""
     import oracle as orc
     threads = cpu_threads()
@@ -145,7 +155,7 @@ def bench_plain(torch, wt, scenes, args):
     flop_per_launch = wt.WT_FLOP_PER_VERTEX_WAVE * V * W
     achieved = flop_per_launch / launch_s / 1e12
     traffic, traffic_shape = load_traffic()
-    roofline = dict(bound="mfma", engine="valu", achieved=round(achieved, 3), peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
+    roofline = dict(bound="valu", achieved=round(achieved, 3), peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
                     frac=round(achieved / PEAK_FP32_TFLOPS, 4), traffic=traffic, traffic_launch_shape=traffic_shape,
                     launch_us=round(launch_s * 1e6, 3),
                     hbm_write_GBs=round(wt.WT_BYTES_PER_VERTEX_PN * V / launch_s / 1e9, 1),

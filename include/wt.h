@@ -199,6 +199,7 @@ wt_status wt_water_displace(wt_ctx* ctx, const wt_water_params* params, const wt
 /* The water program's remaining outputs (water-grid.vert:66-70, 141-158, 193): the
  * varyings its fragment stage reads, formed from the positions and normals
  * wt_water_displace wrote. Each array is optional (NULL skips it). */
+#define WT_MAX_VARYING_VERTICES 8589934080ull /* 2 * (2^32 - 256) */
 typedef struct wt_varying_buffers {
   float* clip_position;        /* float4: viewProjection * (P, 1), the vertex's gl_Position */
   float* ndc_xy_height0;       /* float2: xy / w of viewProjection * (P.x, 0, P.z, 1) */
@@ -207,8 +208,9 @@ typedef struct wt_varying_buffers {
   float* heightmap_sample;     /* float: the heightmap value sampled at P (heightmap_colour.r) */
 } wt_varying_buffers;
 
-/* One HBM-bound pass over vertex_count (at most 2^34) vertices (positions / normals as
- * float3 arrays).
+/* One HBM-bound pass over vertex_count (at most WT_MAX_VARYING_VERTICES) vertices
+ * (positions / normals as float3 arrays). The limit is what one launch covers: two
+ * vertices per work-item and at most 2^32 - 256 work-items per dispatch.
  * view_projection = P V of the camera and view_only_yaw = V of the same camera with
  * pitch 0 (src/render-engine.cpp:278-284), both 4x4 column-major floats as the
  * reference uploads them (wt_camera_matrices gives V and P in double). params supplies

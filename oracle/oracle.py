@@ -106,6 +106,9 @@ def lib():
         L.wto_inverse.argtypes = [dp, dp]
         L.wto_projector.argtypes = [dp, ctypes.c_double, dp, dp, dp, dp, ctypes.c_int]
         L.wto_projector.restype = ctypes.c_int
+        L.wto_projector_variant.argtypes = [dp, ctypes.c_double, ctypes.c_double, ctypes.c_int, dp, dp, dp, dp,
+                                            ctypes.c_int]
+        L.wto_projector_variant.restype = ctypes.c_int
         L.wto_wrap_index.argtypes = [ctypes.c_longlong, ctypes.c_longlong, ctypes.c_int]
         L.wto_wrap_index.restype = ctypes.c_longlong
         L.wto_sample_bilinear.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -162,14 +165,20 @@ def inverse(m):
     return out.reshape(4, 4).T.copy()
 
 
-def projector(cam, amp, max_pts=64):
-    """Returns (status, corners[4,4] (BL, TL, BR, TR), info dict)."""
+def projector(cam, amp, max_pts=64, pad=None, swap_aim_mix=False):
+    """Returns (status, corners[4,4] (BL, TL, BR, TR), info dict). pad / swap_aim_mix
+    plant a mistake (another padding scalar, the aim mix's operands swapped) for tests
+    that show a pin catches it; the defaults restate the reference."""
     c = np.ascontiguousarray(cam, dtype=np.float64)
     corners = np.zeros(16)
     info = np.zeros(13)
     slab = np.zeros(4 * max_pts)
     ndc = np.zeros(4 * max_pts)
-    st = lib().wto_projector(_p(c), float(amp), _p(corners), _p(info), _p(slab), _p(ndc), max_pts)
+    if pad is None and not swap_aim_mix:
+        st = lib().wto_projector(_p(c), float(amp), _p(corners), _p(info), _p(slab), _p(ndc), max_pts)
+    else:
+        st = lib().wto_projector_variant(_p(c), float(amp), 1.2 if pad is None else float(pad), int(bool(swap_aim_mix)),
+                                         _p(corners), _p(info), _p(slab), _p(ndc), max_pts)
     n = int(info[12])
     d = dict(xmin=info[0], xmax=info[1], ymin=info[2], ymax=info[3], projector_pos=info[4:7].copy(),
              projector_pitch=info[7], aim=info[8:11].copy(), raised=bool(info[11]), npoints=n,

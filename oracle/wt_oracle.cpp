@@ -63,13 +63,18 @@ void wto_inverse(const double m[16], double out[16]) {
 // Projector corners (src/render-engine.cpp:797-1042). info (13 doubles):
 // xmin, xmax, ymin, ymax, projector pos xyz, projector pitch, aim xyz, raised, npoints.
 // slab_pts / ndc_pts (may be null) receive up to max_pts points as (x, y, z, w).
-int wto_projector(const double cam[9], double amp, double corners[16], double info[13], double* slab_pts,
-                  double* ndc_pts, int max_pts) {
+// pad / swap_aim_mix: the padding scalar and aim-mix order (1.2 and 0 restate the
+// reference; other values plant a mistake for the tests that show a pin catches it).
+int wto_projector_variant(const double cam[9], double amp, double pad, int swap_aim_mix, double corners[16],
+                          double info[13], double* slab_pts, double* ndc_pts, int max_pts) {
   wto::Camera<double> c = make_camera(cam);
   c.setRotation(cam[7], cam[8]);
   wto::V4<double> co[4];
   wto::ProjectorDebug<double> dbg{};
-  int st = wto::projectorCorners(c, amp, co, &dbg);
+  wto::ProjectorMutation<double> mut;
+  mut.pad = pad;
+  mut.swapAimMix = swap_aim_mix != 0;
+  int st = wto::projectorCorners(c, amp, co, &dbg, &mut);
   if (info) {
     for (int i = 0; i < 13; ++i) info[i] = 0;
     info[12] = (double)dbg.slabPoints.size();
@@ -96,6 +101,11 @@ int wto_projector(const double cam[9], double amp, double corners[16], double in
       ndc_pts[4 * i + 2] = dbg.ndcPoints[i].z; ndc_pts[4 * i + 3] = dbg.ndcPoints[i].w;
     }
   return 0;
+}
+
+int wto_projector(const double cam[9], double amp, double corners[16], double info[13], double* slab_pts,
+                  double* ndc_pts, int max_pts) {
+  return wto_projector_variant(cam, amp, 1.2, 0, corners, info, slab_pts, ndc_pts, max_pts);
 }
 
 long long wto_wrap_index(long long i, long long size, int wrap) { return wto::wrapIndex(i, size, wrap); }

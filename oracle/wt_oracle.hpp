@@ -208,20 +208,29 @@ template <class T> struct ProjectorDebug {
   bool raised;
 };
 
+// Deliberate departures from the reference's projector, used only by tests that show
+// a pin fails when the restatement is wrong: the frustum padding scalar (1.2 at
+// src/render-engine.cpp:828) and the operand order of the aim-point mix (:953).
+template <class T> struct ProjectorMutation {
+  T pad = T(1.2);
+  bool swapAimMix = false;
+};
+
 // Restates the water block of RenderEngine::render, src/render-engine.cpp:797-1042:
 // displaceable slab, padded frustum corners, edge/slab intersections, projector
 // placement and aim, range matrix, four corner rays intersected with y = 0.
 // Returns the corners in the order of waterGridCornerPoints (:1012-1015):
 // [0] bottom-left, [1] top-left, [2] bottom-right, [3] top-right.
 template <class T>
-int projectorCorners(const Camera<T>& cam, T displaceableAmplitude, V4<T> corners[4], ProjectorDebug<T>* dbg = nullptr) {
+int projectorCorners(const Camera<T>& cam, T displaceableAmplitude, V4<T> corners[4], ProjectorDebug<T>* dbg = nullptr,
+                     const ProjectorMutation<T>* mutation = nullptr) {
   const T amp = displaceableAmplitude;  // :803
   const V3<T> up{T(0), T(1), T(0)};
   const M4<T> viewProjection = cam.proj * cam.view;  // :284
   const M4<T> invVP = inverse(viewProjection);       // :286
   V4<T> fc[8] = {{-1, -1, -1, 1}, {-1, -1, 1, 1}, {-1, 1, -1, 1}, {-1, 1, 1, 1},
                  {1, -1, -1, 1},  {1, -1, 1, 1},  {1, 1, -1, 1},  {1, 1, 1, 1}};  // :815-822
-  const T PAD = T(1.2);                                                            // :828
+  const T PAD = mutation ? mutation->pad : T(1.2);                                // :828
   for (auto& p : fc) { p.x *= PAD; p.y *= PAD; }                                   // :829-832
   for (auto& p : fc) { V4<T> q = invVP * p; p = q / q.w; }                         // :835-838
   static const int E[24] = {0, 1, 0, 2, 0, 4, 1, 3, 1, 5, 2, 3, 2, 6, 3, 7, 4, 5, 4, 6, 5, 7, 6, 7};  // :842-853
@@ -264,7 +273,7 @@ int projectorCorners(const Camera<T>& cam, T displaceableAmplitude, V4<T> corner
   V3<T> aim2 = cam.position + cam.forward * FWDLEN;  // :944
   aim2.y = T(0);                              // :946
   const T a = std::fabs(fwdSafe.y);           // :950
-  const V3<T> aim = mix(aim2, aim1, a);       // :953
+  const V3<T> aim = (mutation && mutation->swapAimMix) ? mix(aim1, aim2, a) : mix(aim2, aim1, a);  // :953
   const V3<T> nf = normalize(aim - projector.position);              // :956
   const V3<T> nfxz = normalize(V3<T>{nf.x, T(0), nf.z});             // :957
   T pitchDeg = -degrees(std::acos(clamp(dot(nf, nfxz), T(-1), T(1))));  // :959

@@ -11,6 +11,13 @@ Fixtures (little-endian, numpy .npz without pickles, plus a JSON header):
   w4_t10_n17.npz      the 256x256 scene's waves (4 seeded waves, t = 10 s) on a 17x17 grid, N/T/B
   bilinear.npz        64 samples of a 16x16 R8 texture under REPEAT and MIRRORED_REPEAT
   adjoint_small.npz   loss and gradient of a 16x16 x 2 snapshots x 3 waves fit
+  waves3_00_r8.npz    the reference's own heightmap asset, assets/textures/noise/waves/
+                      waves3/00.png (512 x 512 grey), decoded as RenderEngine::load2DTexture
+                      does it (src/render-engine.cpp:1308-1322: stb_image forcing RGBA,
+                      rows flipped so row 0 is the image's bottom), red channel kept as the
+                      R8 texels the water program samples; decoded with PIL (PNG is
+                      lossless, so any decoder gives these bytes)
+Run with fixture names as arguments to rewrite only those.
 """
 import json
 import math
@@ -94,19 +101,49 @@ def make_adjoint():
         n=16, S=2, W=3, spacing=prob["spacing"], x0=prob["x0"], z0=prob["z0"], t0=prob["t0"], dt=prob["dt"])
 
 
+REFERENCE_ASSET = "/root/reference/assets/textures/noise/waves/waves3/00.png"
+
+
+def stb_like_red(img):
+    """What stb_image with STBI_rgb_alpha and vertical flip hands the reference's texture
+    upload, red channel only: grey g -> (g, g, g, 255), palette -> its RGB, 16-bit -> the
+    high byte; rows flipped so row 0 is the image's bottom."""
+    if img.mode in ("I;16", "I;16B"):
+        a = (np.asarray(img, dtype=np.uint16) >> 8).astype(np.uint8)
+    else:
+        a = np.asarray(img.convert("RGBA"))[:, :, 0]
+    return a[::-1].copy()
+
+
+def make_waves3_asset():
+    import hashlib
+    from PIL import Image
+    raw = open(REFERENCE_ASSET, "rb").read()
+    img = Image.open(REFERENCE_ASSET)
+    tex = stb_like_red(img)
+    return dict(texels=tex), dict(source="assets/textures/noise/waves/waves3/00.png", mode=img.mode,
+                                  width=int(img.size[0]), height=int(img.size[1]),
+                                  png_sha256=hashlib.sha256(raw).hexdigest(),
+                                  texels_sha256=hashlib.sha256(tex.tobytes()).hexdigest())
+
+
 MAKERS = {"default_scene": make_default_scene, "w4_t10_n17": make_w4_t10, "bilinear": make_bilinear,
-          "adjoint_small": make_adjoint}
+          "adjoint_small": make_adjoint, "waves3_00_r8": make_waves3_asset}
 
 
-def main():
-    header = {}
+def main(names):
+    path = os.path.join(HERE, "golden.json")
+    header = json.load(open(path)) if names and os.path.exists(path) else {}
     for name, fn in MAKERS.items():
+        if names and name not in names:
+            continue
         arrays, meta = fn()
-        np.savez(os.path.join(HERE, name + ".npz"), **arrays)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays) if name == "waves3_00_r8" else \
+            np.savez(os.path.join(HERE, name + ".npz"), **arrays)
         header[name] = meta
-    with open(os.path.join(HERE, "golden.json"), "w") as f:
+    with open(path, "w") as f:
         json.dump(header, f, indent=1, sort_keys=True)
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -59,11 +59,12 @@ def test_gradient_matches_fp64_adjoint(torch_cuda, wtctx, n, S, W):
     fit.close()
 
 
-def test_batched_equals_single(torch_cuda, wtctx):
-    """One batched launch over uneven problems gives bit-identical gradients to the
-    per-problem launches (same kernels, same fixed reduction order)."""
+def test_batched_matches_oracle_and_single(torch_cuda, wtctx):
+    """One batched launch over uneven problems: every problem's loss and gradient against
+    the FP64 adjoint, and bit-identical to that problem's own single-problem launch (same
+    kernels, same fixed reduction order)."""
     torch = torch_cuda
-    say("batched vs single")
+    say("batched vs oracle and single")
     specs = [(23, 4, 8), (64, 9, 12), (5, 1, 3), (100, 16, 20), (33, 32, 64)]
     probs = [make_problem(torch, 300 + i, *s) for i, s in enumerate(specs)]
     total = sum(s[2] * 5 for s in specs)
@@ -72,10 +73,16 @@ def test_batched_equals_single(torch_cuda, wtctx):
     fitb = wt.Fit(wtctx, [p[4] for p in probs])
     fitb.loss_grad(gb, lb)
     off = 0
+    torch.cuda.synchronize()
     for i, (p, s) in enumerate(zip(probs, specs)):
+        prob, tgt, t_target, t_params, fp = p
+        L_ref, g_ref = oracle_grad(prob, tgt, t_params.cpu().numpy().astype(np.float64))
+        g = gb[off:off + s[2] * 5].cpu().numpy().reshape(-1, 5)
+        assert np.abs(g - g_ref).max() <= GRAD_RTOL * np.abs(g_ref).max(), i
+        assert abs(lb[i].item() - L_ref) <= 1e-5 * L_ref, i
         g1 = torch.zeros(s[2] * 5, device="cuda")
         l1 = torch.zeros(1, device="cuda")
-        f1 = wt.Fit(wtctx, [p[4]])
+        f1 = wt.Fit(wtctx, [fp])
         f1.loss_grad(g1, l1)
         torch.cuda.synchronize()
         assert torch.equal(g1, gb[off:off + s[2] * 5]), i

@@ -134,6 +134,22 @@ def test_batch_rejects_bad_input(torch_cuda, wtctx):
     with pytest.raises(wt.WtError) as e:
         wt.Batch(wtctx, bad)
     assert e.value.status == wt.WT_ERR_INVALID_ARG
+    # a wave range that wraps 32 bits, and a vertex offset past the 2^48 limit
+    patches, waves, V = wt.batch_inputs(specs)
+    patches[2].wave_offset = 2 ** 32 - 2
+    with pytest.raises(wt.WtError) as e:
+        wt.Batch(wtctx, patches)
+    assert e.value.status == wt.WT_ERR_INVALID_ARG
+    patches, waves, V = wt.batch_inputs(specs)
+    patches[0].wave_offset = 2 ** 31 - 2
+    with pytest.raises(wt.WtError) as e:
+        wt.Batch(wtctx, patches)
+    assert e.value.status == wt.WT_ERR_INVALID_ARG
+    patches, waves, V = wt.batch_inputs(specs)
+    patches[1].vertex_offset = 2 ** 48 + 1
+    with pytest.raises(wt.WtError) as e:
+        wt.Batch(wtctx, patches)
+    assert e.value.status == wt.WT_ERR_INVALID_ARG
     patches, waves, V = wt.batch_inputs(specs)
     b = wt.Batch(wtctx, patches)
     short = torch_cuda.from_numpy(waves[:2]).cuda()
@@ -142,3 +158,61 @@ def test_batch_rejects_bad_input(torch_cuda, wtctx):
         b.displace(short, pos, pos.clone())
     assert e.value.status == wt.WT_ERR_INVALID_ARG
     b.close()
+
+
+def moved_camera(c, f):
+    """Camera c after f frames of a slow fly-by: yaw turns 0.05 deg, the eye moves 0.1 m
+    along x and -0.05 m along z and rises 0.2% per frame (float32-representable)."""
+    return scenes.camera_tuple(height=c[5] * (1.0 + 0.002 * f), pitch_deg=c[8], yaw_deg=c[7] + 0.05 * f,
+                               fov_deg=c[0], aspect=c[1], near=c[2], far=c[3], x=c[4] + 0.1 * f, z=c[6] - 0.05 * f)
+
+
+def test_config2_full_batch_reaimed_on_device(torch_cuda, wtctx):
+    """The 512-patch batch of the benchmark (seed 2: sides 32-1024, 4-128 waves, t in
+    [0, 3600) s) re-aimed by wt_batch_set_cameras and displaced at frames 0 and 119 of
+    its 120-frame loop, the cameras moving between the two. 18 patches per frame -
+    the largest grid, the most waves and 16 drawn at random - against the FP64 oracle,
+    whose corners come from the oracle's own projector for the same cameras."""
+    torch = torch_cuda
+    dev = torch.device("cuda")
+    specs = scenes.batched_patches(seed=2, count=512)
+    patches, waves_np, V = wt.batch_inputs(specs)
+    batch = wt.Batch(wtctx, patches)
+    waves = torch.from_numpy(waves_np).to(dev)
+    texels = scenes.value_noise(1, 512)
+    hm = torch.from_numpy(texels).to(dev)
+    pos = torch.full((V * 3,), float("nan"), device=dev)
+    nrm = torch.full((V * 3,), float("nan"), device=dev)
+    amps = torch.full((len(specs),), 2.0, dtype=torch.float32, device=dev)
+    status = torch.full((len(specs),), -1, dtype=torch.int32, device=dev)
+    rng = np.random.default_rng(22)
+    big = max(range(len(specs)), key=lambda i: specs[i]["N"])
+    many = max(range(len(specs)), key=lambda i: specs[i]["W"])
+    for f in (0, 119):
+        cams = [moved_camera(s["camera"], f) for s in specs]
+        batch.set_cameras(torch.from_numpy(wt.camera_rows(cams)).to(dev), amps, status)
+        batch.displace(waves, pos, nrm, heightmap=hm, time_offset=f / 60.0)
+        torch.cuda.synchronize()
+        st = status.cpu().numpy()
+        assert (st == wt.WT_OK).all(), np.unique(st)
+        pick = [big, many] + [int(i) for i in rng.choice(len(specs), size=16, replace=False)]
+        say("config 2 frame %d: %d patches, checking %s" % (f, len(specs), pick))
+        for i in pick:
+            s, p = specs[i], patches[i]
+            N = s["N"]
+            ost, corners, _ = orc.projector(cams[i], 2.0)
+            assert ost == 0
+            t = s["time"] + f / 60.0
+            sd = orc.SceneData(corners, cams[i][4:7], N, waves_np[p.wave_offset:p.wave_offset + p.wave_count], t,
+                               0.02, 0.05, 0.0, (0.3, 0.0), texels)
+            ref = orc.vertices(sd, nthreads=NTHREADS, want=("pos", "nrm"))
+            o = p.vertex_offset * 3
+            out = dict(pos=pos[o:o + N * N * 3].cpu().numpy().reshape(-1, 3),
+                       nrm=nrm[o:o + N * N * 3].cpu().numpy().reshape(-1, 3))
+            edge = near_texel_edge(ref["pos"], 0.02, (0.3, 0.0), t, 512, 512)
+            res = compare(out, ref, cams[i][4:7], edge)
+            print(f, i, N, s["W"], res, flush=True)
+            assert res["n_near"] > 0
+            assert res["pos_max"] <= POS_TOL and res["nrm_max"] <= NRM_TOL, (f, i)
+            assert res["n_excluded"] <= 0.02 * res["n_near"] + 4
+    batch.close()

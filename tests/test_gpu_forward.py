@@ -84,12 +84,31 @@ def test_grid256_w4_parity(torch_cuda, wtctx):
     assert res["tan_max"] <= NRM_TOL and res["bit_max"] <= NRM_TOL
 
 
+BENCH_SHAPE = (2, 16, 64)  # the shape bench.py reports for this workload (wave_split, steps, lanes per row)
+
+
 @pytest.mark.parametrize("frame", [0, 239])
-def test_grid1024_w64_heightmap_frame_parity(torch_cuda, wtctx, frame):
-    """1024^2 grid, 64 waves, 512^2 FP32 value-noise heightmap (REPEAT, scrolled)."""
-    say("1024x1024 grid, W=64, heightmap, frame %d" % frame)
+@pytest.mark.parametrize("shape", ["default", "bench", "autotuned"])
+def test_grid1024_w64_heightmap_frame_parity(torch_cuda, wtctx, frame, shape):
+    """1024^2 grid, 64 waves, 512^2 FP32 value-noise heightmap (REPEAT, scrolled), at the
+    first and last frame of the 240-frame step, with the context's default launch shape,
+    the shape the benchmark reports, and whatever the autotuner picks on this device."""
     cfg, corners, waves, cam_pos, t, texels = scene_setup(scenes.scene_1024_w64_heightmap(), frame)
-    res = check_parity(torch_cuda, wtctx, cfg, corners, waves, cam_pos, t, texels, False)
+    ctx = wtctx
+    if shape == "bench":
+        wtctx.set_launch_config(*BENCH_SHAPE, 64)
+    elif shape == "autotuned":
+        ctx = wt.Context()
+        ctx.autotune(cfg["N"], cfg["W"], True)
+    say("1024x1024 grid, W=64, heightmap, frame %d, %s shape %s" % (frame, shape,
+                                                                     ctx.get_launch_config(cfg["N"], cfg["W"])))
+    try:
+        res = check_parity(torch_cuda, ctx, cfg, corners, waves, cam_pos, t, texels, False)
+    finally:
+        if shape == "bench":
+            wtctx.set_launch_config()
+        elif shape == "autotuned":
+            ctx.close()
     print(res, flush=True)
     assert res["pos_max"] <= POS_TOL
     assert res["nrm_max"] <= NRM_TOL
@@ -326,8 +345,11 @@ def test_invalid_arguments(torch_cuda, wtctx):
     with pytest.raises(wt.WtError) as e:
         wtctx.displace(ok, waves, host, nrm)
     assert e.value.status == wt.WT_ERR_INVALID_ARG
-    with pytest.raises(wt.WtError) as e:
-        wtctx.displace(ok, waves, pos, nrm, wave_count=wt._abi.WT_ERR_INVALID_ARG + 5000)
+    with pytest.raises(ValueError):  # more waves than the array holds: refused before the ABI
+        wtctx.displace(ok, waves, pos, nrm, wave_count=5)
+    many = torch.zeros((wt._abi.WT_MAX_WAVES + 1, 8), device=dev)
+    with pytest.raises(wt.WtError) as e:  # beyond the ABI's wave limit
+        wtctx.displace(ok, many, pos, nrm)
     assert e.value.status == wt.WT_ERR_INVALID_ARG
     hm = torch.zeros((8, 8), device=dev)
     with pytest.raises(wt.WtError) as e:

@@ -109,3 +109,13 @@ def test_varyings_reject_bad_arguments(torch_cuda, wtctx):
         wtctx.varyings(params, bad, eye, pos, pos, 16, view=torch.empty(48, device=dev))
     # nothing requested: a no-op
     wtctx.varyings(params, eye, eye, pos, pos, 16)
+    # one past what one launch covers is refused by the ABI (called directly: the wrapper
+    # would refuse the undersized buffers first)
+    import ctypes
+    f16 = ctypes.c_float * 16
+    m = f16(*[float(v) for v in eye.reshape(-1)])
+    out = wt._abi.VaryingBuffers(None, None, ctypes.c_void_p(pos.data_ptr()), None, None)
+    st = wt.lib().wt_vertex_varyings(wtctx.handle, ctypes.byref(params), None, m, m, ctypes.c_void_p(pos.data_ptr()),
+                                     ctypes.c_void_p(pos.data_ptr()), wt._abi.WT_MAX_VARYING_VERTICES + 1,
+                                     ctypes.byref(out), None)
+    assert st == wt.WT_ERR_INVALID_ARG

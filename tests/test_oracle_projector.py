@@ -148,3 +148,116 @@ def test_library_camera_matrices_match_oracle():
         m = orc.camera(c)
         assert np.allclose(v, m["view"], atol=1e-12) and np.allclose(p, m["proj"], atol=1e-12)
         assert np.allclose(f, m["forward"], atol=1e-12)
+
+
+# ------------------------------------------------------------------ closed-form pins
+# These pin the two free constants of the projector, the frustum padding scalar
+# (src/render-engine.cpp:828-832) and the aim-point blend with its horizon epsilon
+# (:920-953), by facts that a wrong constant or a swapped mix violates. Each pin is
+# also run against the oracle with that mistake planted, and must fail there.
+AIM_EPS = 0.001  # render-engine.cpp:921
+PAD_PROBE = 1.19  # NDC magnitude inside the padded frustum (1.2) and outside the unpadded one
+
+
+def quad_margin(corners, pts):
+    """Signed distance (m) of xz points inside the convex corner quad BL, BR, TR, TL:
+    positive inside."""
+    poly = corners[[0, 2, 3, 1]][:, [0, 2]]
+    area = sum(cross2(poly[i], poly[(i + 1) % 4]) for i in range(4))
+    sgn = 1.0 if area > 0 else -1.0
+    margin = np.full(len(pts), np.inf)
+    for i in range(4):
+        a, b = poly[i], poly[(i + 1) % 4]
+        e = b - a
+        d = pts - a
+        margin = np.minimum(margin, sgn * (e[0] * d[:, 1] - e[1] * d[:, 0]) / np.linalg.norm(e))
+    return margin
+
+
+def padded_frustum_hits(c, lim=PAD_PROBE, n=49):
+    """Where camera rays through the NDC border |x| = lim or |y| = lim meet y = 0 between
+    the near and the far plane (xz points)."""
+    m = orc.camera(c)
+    inv = np.linalg.inv(m["proj"] @ m["view"])
+    ts = np.linspace(-lim, lim, n)
+    ndc = [(sx * lim, t) for sx in (-1, 1) for t in ts] + [(t, sy * lim) for sy in (-1, 1) for t in ts]
+    out = []
+    for x, y in ndc:
+        a = inv @ np.array([x, y, -1.0, 1.0])
+        b = inv @ np.array([x, y, 1.0, 1.0])
+        a, b = a[:3] / a[3], b[:3] / b[3]
+        if (a[1] > 0) != (b[1] > 0):
+            p = a + (a[1] / (a[1] - b[1])) * (b - a)
+            out.append((p[0], p[2]))
+    return np.array(out)
+
+
+PIN_CAMERAS = [
+    cam(),                                                                   # the benchmark camera
+    cam(height=4.0, pitch=0.0, yaw=-90.0, fov=72.0, aspect=1.0, far=100.0, z=70.0),  # reference default (:52)
+    cam(height=1.0, pitch=-10.0, yaw=30.0),                                  # projector raised above the slab
+    cam(height=25.0, pitch=-60.0, yaw=135.0, fov=45.0),                      # steep
+    cam(height=10.0, pitch=5.0, yaw=-20.0, far=500.0),                       # looking up
+    cam(height=40.0, pitch=-30.0, yaw=250.0, fov=90.0, aspect=2.0, far=300.0),
+]
+
+
+@pytest.mark.parametrize("ci", range(len(PIN_CAMERAS)))
+def test_padded_frustum_lies_inside_the_corner_quad(ci):
+    """Every point of y = 0 that the padded frustum sees lies inside the corner quad: the
+    range window bounds the padded frustum's slab points, so rays through NDC +-1.19 (inside
+    the x1.2 padding) meet the plane inside the grid. Without the padding, or with 1.15,
+    some of them fall outside."""
+    c = PIN_CAMERAS[ci]
+    pts = padded_frustum_hits(c)
+    assert len(pts) >= 40
+    st, corners, _ = orc.projector(c, 2.0)
+    assert st == 0
+    assert quad_margin(corners, pts).min() > 0.0
+    for wrong in (1.0, 1.15):
+        st, corners_w, _ = orc.projector(c, 2.0, pad=wrong)
+        assert st == 0 and quad_margin(corners_w, pts).min() < 0.0, wrong
+
+
+def aim_closed_form(c, amp):
+    """Projector pitch and aim point in closed form for a camera above the slab's base
+    plane: forward f = (cos p cos y, sin p, cos p sin y), a = max(|sin p|, eps).
+    aim_1 meets y = 0 along f with its y clamped to +-a (mirrored when looking up), at
+    horizontal distance h cos p / a; aim_2 is f's horizontal part (length cos p). The
+    blend a aim_1 + (1 - a) aim_2 therefore sits (1 - a) cos p + h cos p ahead, and the
+    projector, at height max(h, amp + 1), pitches down by atan(height / distance)."""
+    h, yaw, p = c[5], math.radians(c[7]), math.radians(c[8])
+    a = max(abs(math.sin(p)), AIM_EPS)
+    dist = (1.0 - a) * math.cos(p) + h * math.cos(p)
+    hp = max(h, amp + 1.0)
+    aim = np.array([c[4] + dist * math.cos(yaw), 0.0, c[6] + dist * math.sin(yaw)])
+    return -math.degrees(math.atan2(hp, dist)), aim
+
+
+@pytest.mark.parametrize("h,pitch,yaw", [(10.0, 0.0, -90.0), (4.0, 0.0, 45.0), (10.0, -15.0, 30.0),
+                                         (1.0, -10.0, 200.0), (25.0, -60.0, 10.0), (10.0, 5.0, 77.0),
+                                         (6.0, 0.005, 0.0), (6.0, -0.005, 123.0)])
+def test_projector_aim_closed_form(h, pitch, yaw):
+    """At pitch 0 the aim point is 1 - eps + h ahead and the projector pitch is
+    -atan(h_p / (h + 1 - eps)); other pitches, the raised projector and the clamped
+    horizon case follow the same closed form. Swapping the mix's operands (the aim then
+    sits h / eps ahead at the horizon) fails it."""
+    amp = 2.0
+    c = cam(height=h, pitch=pitch, yaw=yaw)
+    want_pitch, want_aim = aim_closed_form(c, amp)
+    st, _, info = orc.projector(c, amp)
+    assert st == 0
+    assert abs(info["projector_pitch"] - want_pitch) < 1e-9
+    assert np.abs(info["aim"] - want_aim).max() < 1e-9 * max(1.0, np.abs(want_aim).max())
+    assert abs(info["projector_pos"][1] - max(c[5], amp + 1.0)) < 1e-12
+    st, _, bad = orc.projector(c, amp, swap_aim_mix=True)
+    assert st == 0 and abs(bad["projector_pitch"] - want_pitch) > 1.0
+
+
+def test_horizon_aim_distance_at_pitch_zero():
+    """The pitch-0 case written out: h = 10 m, eps = 0.001: the aim point is 10.999 m
+    ahead and the projector looks down atan(10 / 10.999)."""
+    st, _, info = orc.projector(cam(height=10.0, pitch=0.0, yaw=-90.0), 2.0)
+    assert st == 0
+    assert abs(info["aim"][2] - (-(1.0 - AIM_EPS + 10.0))) < 1e-9 and abs(info["aim"][0]) < 1e-9
+    assert abs(info["projector_pitch"] + math.degrees(math.atan(10.0 / 10.999))) < 1e-9

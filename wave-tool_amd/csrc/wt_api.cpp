@@ -255,7 +255,8 @@ wt_status wt_vertex_varyings(wt_ctx* ctx, const wt_water_params* p, const wt_hei
                              const wt_varying_buffers* out, void* stream) {
   if (!ctx) return WT_ERR_INVALID_ARG;
   if (!p || !out || !vp || !vyaw) return fail(ctx, WT_ERR_INVALID_ARG, "params, matrices and out must be non-NULL");
-  if (count > (1ull << 34)) return fail(ctx, WT_ERR_INVALID_ARG, "vertex_count too large (limit 2^34)");
+  if (count > WT_MAX_VARYING_VERTICES)
+    return fail(ctx, WT_ERR_INVALID_ARG, "vertex_count too large (limit 2 (2^32 - 256), one launch)");
   for (int i = 0; i < 16; ++i)
     if (!std::isfinite(vp[i]) || !std::isfinite(vyaw[i])) return fail(ctx, WT_ERR_INVALID_ARG, "matrices must be finite");
   const float fs[6] = {p->cameraPosition[0], p->cameraPosition[1], p->cameraPosition[2], p->heightmapSampleScale,
@@ -502,6 +503,9 @@ struct wt_batch {
   uint32_t max_wave_end = 0;
 };
 
+// Largest patch vertex_offset a batch accepts (2^48 float3 = 3 PB, beyond any device).
+static constexpr uint64_t kMaxBatchVertexOffset = 1ull << 48;
+
 // forced_shape < 0: each patch's shape from the context's batch autotune table, else
 // the padded-vertex model (batch_shape_for)
 static wt_status batch_create(wt_ctx* ctx, const wt_patch* patches, uint32_t count, int forced_shape,
@@ -516,8 +520,11 @@ static wt_status batch_create(wt_ctx* ctx, const wt_patch* patches, uint32_t cou
   for (uint32_t i = 0; i < count; ++i) {
     const wt_patch& p = patches[i];
     const std::string tag = "patch " + std::to_string(i) + ": ";
+    // wave and vertex ranges checked in 64 bits: the kernels index with int wave offsets
+    // and 64-bit vertex offsets, so a sum that wraps must not pass
     bool ok = p.gridLength >= 2 && p.gridLength <= 32768 && p.wave_count <= WT_MAX_WAVES &&
-              std::isfinite(p.waveAnimationTimeInSeconds);
+              (uint64_t)p.wave_offset + p.wave_count <= (uint64_t)INT32_MAX &&
+              p.vertex_offset <= kMaxBatchVertexOffset && std::isfinite(p.waveAnimationTimeInSeconds);
     for (int r = 0; r < 4 && ok; ++r)
       for (int k = 0; k < 4; ++k) ok = ok && std::isfinite(p.corners[r][k]);
     const float fs[8] = {p.cameraPosition[0], p.cameraPosition[1], p.cameraPosition[2], p.heightmapSampleScale,
@@ -526,7 +533,9 @@ static wt_status batch_create(wt_ctx* ctx, const wt_patch* patches, uint32_t cou
     for (float f : fs) ok = ok && std::isfinite(f);
     if (!ok) {
       delete b;
-      return fail(ctx, WT_ERR_INVALID_ARG, tag + "gridLength in [2, 32768], wave_count <= WT_MAX_WAVES, finite values");
+      return fail(ctx, WT_ERR_INVALID_ARG, tag + "gridLength in [2, 32768], wave_count <= WT_MAX_WAVES, "
+                                                 "wave_offset + wave_count <= 2^31 - 1, vertex_offset <= 2^48, "
+                                                 "finite values");
     }
     wt::PatchDesc d{};
     wt::set_corner_algebra(d, p.corners);

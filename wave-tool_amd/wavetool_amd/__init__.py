@@ -116,6 +116,39 @@ def _check_array(t, dtype, numel, name):
                          (name, dtype, numel, t.dtype, tuple(t.shape)))
 
 
+def _check_min_array(t, dtype, numel, name):
+    """A tensor the kernels index up to numel entries of (a view into a larger buffer
+    is fine): refuse a wrong dtype, a non-contiguous view or one that holds fewer."""
+    if t.dtype != dtype or not t.is_contiguous() or t.numel() < numel:
+        raise ValueError("%s: need a contiguous %s tensor of at least %d elements, got %s %s" %
+                         (name, dtype, numel, t.dtype, tuple(t.shape)))
+
+
+def _check_waves(waves, count):
+    import torch
+    if count and (waves is None or waves.dim() != 2 or waves.shape[1] != 8 or waves.shape[0] < count):
+        raise ValueError("waves: need a (>= %d, 8) tensor, got %s" % (count, None if waves is None
+                                                                         else tuple(waves.shape)))
+    if count:
+        _check_min_array(waves, torch.float32, count * 8, "waves")
+
+
+def _heightmap(heightmap, hm_format, hm_wrap):
+    """wt_heightmap over a 2-D device tensor whose dtype matches the texel format
+    (uint8 for WT_TEXEL_R8_UNORM, float32 for WT_TEXEL_R32F)."""
+    import torch
+    if heightmap is None:
+        return None
+    if heightmap.dim() != 2:
+        raise ValueError("heightmap: need a 2-D (height, width) tensor, got %s" % (tuple(heightmap.shape),))
+    want = {WT_TEXEL_R8_UNORM: torch.uint8, WT_TEXEL_R32F: torch.float32}.get(hm_format)
+    if want is None:
+        raise ValueError("heightmap: unknown texel format %r" % (hm_format,))
+    h, w = heightmap.shape
+    _check_array(heightmap, want, h * w, "heightmap")
+    return Heightmap(heightmap.data_ptr(), w, h, hm_format, hm_wrap)
+
+
 def _stream(stream):
     if stream is None:
         import torch
@@ -184,12 +217,17 @@ class Context:
                  hm_wrap=WT_WRAP_REPEAT, stream=None, wave_count=None):
         """waves: device tensor (W, 8) float32; pos/nrm/tan/bit: device float tensors
         with gridLength^2 * 3 elements; heightmap: device tensor (H, W) or None."""
+        import torch
         W = int(waves.shape[0]) if wave_count is None else int(wave_count)
+        _check_waves(waves, W)
+        n3 = int(params.gridLength) ** 2 * 3
+        for name, t in (("pos", pos), ("nrm", nrm), ("tan", tan), ("bit", bit)):
+            if t is not None or name in ("pos", "nrm"):
+                if t is None:
+                    raise ValueError("%s is required" % name)
+                _check_min_array(t, torch.float32, n3, name)
         vb = VertexBuffers(_ptr(pos), _ptr(nrm), _ptr(tan), _ptr(bit))
-        hm = None
-        if heightmap is not None:
-            h, w = heightmap.shape
-            hm = Heightmap(heightmap.data_ptr(), w, h, hm_format, hm_wrap)
+        hm = _heightmap(heightmap, hm_format, hm_wrap)
         st = lib().wt_water_displace(self.handle, ctypes.byref(params), _ptr(waves) if W else None, W,
                                      ctypes.byref(hm) if hm is not None else None, ctypes.byref(vb), _stream(stream))
         self._check(st)
@@ -201,14 +239,17 @@ class Context:
         Matrices: 4x4 arrays in the usual row / column indexing (m[r][c]); they are
         passed column-major as the reference uploads them."""
         import numpy as np
+        import torch
+        n = int(vertex_count)
+        for name, t, k in (("pos", pos, 3), ("nrm", nrm, 3), ("clip", clip, 4), ("ndc0", ndc0, 2), ("view", view, 3),
+                           ("nview", nview, 3), ("sample", sample, 1)):
+            if t is not None:
+                _check_min_array(t, torch.float32, n * k, name)
         f16 = ctypes.c_float * 16
         vp = f16(*[float(v) for v in np.asarray(view_projection, dtype=np.float64).T.reshape(-1)])
         vy = f16(*[float(v) for v in np.asarray(view_only_yaw, dtype=np.float64).T.reshape(-1)])
         out = VaryingBuffers(_ptr(clip), _ptr(ndc0), _ptr(view), _ptr(nview), _ptr(sample))
-        hm = None
-        if heightmap is not None:
-            h, w = heightmap.shape
-            hm = Heightmap(heightmap.data_ptr(), w, h, hm_format, hm_wrap)
+        hm = _heightmap(heightmap, hm_format, hm_wrap)
         st = lib().wt_vertex_varyings(self.handle, ctypes.byref(params), ctypes.byref(hm) if hm is not None else None,
                                       vp, vy, _ptr(pos), _ptr(nrm), int(vertex_count), ctypes.byref(out),
                                       _stream(stream))
@@ -311,11 +352,16 @@ class Batch:
 
     def displace(self, waves, pos, nrm, tan=None, bit=None, heightmap=None, hm_format=WT_TEXEL_R32F,
                  hm_wrap=WT_WRAP_REPEAT, time_offset=0.0, stream=None):
+        import torch
+        _check_waves(waves, int(waves.shape[0]))
+        n3 = int(self.vertices) * 3
+        for name, t in (("pos", pos), ("nrm", nrm), ("tan", tan), ("bit", bit)):
+            if t is not None or name in ("pos", "nrm"):
+                if t is None:
+                    raise ValueError("%s is required" % name)
+                _check_min_array(t, torch.float32, n3, name)
         vb = VertexBuffers(_ptr(pos), _ptr(nrm), _ptr(tan), _ptr(bit))
-        hm = None
-        if heightmap is not None:
-            h, w = heightmap.shape
-            hm = Heightmap(heightmap.data_ptr(), w, h, hm_format, hm_wrap)
+        hm = _heightmap(heightmap, hm_format, hm_wrap)
         st = lib().wt_water_displace_batched(self.ctx.handle, self.handle, _ptr(waves), int(waves.shape[0]),
                                              ctypes.byref(hm) if hm is not None else None, ctypes.byref(vb),
                                              float(time_offset), _stream(stream))

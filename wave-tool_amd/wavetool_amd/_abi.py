@@ -27,6 +27,8 @@ WT_FLOP_PER_VERTEX_WAVE = 26
 WT_BYTES_PER_VERTEX_PN = 24
 WT_BYTES_PER_VERTEX_PNTB = 48
 WT_ADJOINT_FLOP_PER_VERTEX_WAVE_SNAPSHOT = 50
+WT_MAX_WAVES = 1024
+WT_MAX_VARYING_VERTICES = 2 * (2 ** 32 - 256)
 
 
 class GerstnerWave(ctypes.Structure):
