@@ -12,7 +12,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# WT_ORACLE_LIB names another build of the oracle (the sanitizer build)
+LIB_PATH = os.environ.get("WT_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 MODE_ANALYTIC = 0
 MODE_REFERENCE = 1

@@ -12,7 +12,7 @@ import wavetool_amd as wt
 from helpers import NRM_TOL, POS_TOL, compare, near_texel_edge, reference_indices, reference_indices_vectorized
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-CHECK = os.path.join(ROOT, "wave-tool_amd", "host", "water_stage_check")
+CHECK = os.environ.get("WT_STAGE_CHECK") or os.path.join(ROOT, "wave-tool_amd", "host", "water_stage_check")
 GOLDEN = os.path.join(ROOT, "tests", "golden", "default_scene.npz")
 
 

@@ -17,7 +17,7 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLDEN = os.path.join(ROOT, "tests", "golden")
-CHECK = os.path.join(ROOT, "wave-tool_amd", "host", "water_stage_check")
+CHECK = os.environ.get("WT_STAGE_CHECK") or os.path.join(ROOT, "wave-tool_amd", "host", "water_stage_check")
 ASSET = "/root/reference/assets/textures/noise/waves/waves3/00.png"
 
 

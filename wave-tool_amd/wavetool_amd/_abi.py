@@ -8,7 +8,9 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libwavetool.so")
+# WT_LIBWAVETOOL names another build of the library (the host-sanitizer build of the
+# CPU test run); the default is the in-tree build
+LIB_PATH = os.environ.get("WT_LIBWAVETOOL") or os.path.join(HERE, "libwavetool.so")
 
 WT_OK = 0
 WT_ERR_INVALID_ARG = 1
