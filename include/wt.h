@@ -421,6 +421,18 @@ wt_status wt_fit_iterate(wt_fit* fit, const wt_adam_config* adam, uint32_t itera
 /* Loss and gradient at the current params: d_grad receives the problems' W*5
  * gradients concatenated in batch order, d_losses (may be NULL) one loss each. */
 wt_status wt_fit_loss_grad(wt_fit* fit, float* d_grad, float* d_losses, void* stream);
+/* wt_fit_loss_grad for one problem without a wt_fit and without allocating: no
+ * counterpart in the reference; for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). The
+ * descriptor, maps and per-block partial sums live in a fixed device array of
+ * WT_FIT_STATIC_SCRATCH_BYTES per GPU, held until the process ends; a problem needing
+ * more (about (5 W + 1) x ceil(n^2 / 256) x 4 bytes of partials, half the blocks for
+ * S <= 8) is refused with WT_ERR_INVALID_ARG. d_grad receives W*5 floats, d_loss (may be
+ * NULL) one. Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+ * (on different streams or host threads) race and must be ordered by the caller. */
+#define WT_FIT_STATIC_SCRATCH_BYTES (16u << 20)
+wt_status wt_fit_loss_grad_static(wt_ctx* ctx, const wt_fit_problem* problem, float* d_grad, float* d_loss,
+                                  void* stream);
 /* Writes the displacement of problem->params (taken as the hidden wave set) at every
  * grid point and snapshot into d_target, plus N(0, sigma) noise per component from a
  * counter-based generator keyed by seed (problem->target is ignored). */

@@ -514,6 +514,26 @@ hipError_t launch_zero_words(uint32_t* p, size_t n, hipStream_t st) {
   return hipGetLastError();
 }
 
+__global__ void fit_setup_single_kernel(FitDesc d, FitDesc* __restrict__ d_desc, int* __restrict__ d_maps,
+                                        size_t map_words, uint32_t* __restrict__ zero_begin, size_t zero_words) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const size_t i0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i0 == 0) *d_desc = d;
+  for (size_t i = i0; i < map_words; i += stride) d_maps[i] = 0;
+  for (size_t i = i0; i < zero_words; i += stride) zero_begin[i] = 0u;
+}
+
+hipError_t launch_fit_setup_single(const FitDesc& d, FitDesc* d_desc, int* d_maps, size_t map_words,
+                                   uint32_t* zero_begin, size_t zero_words, hipStream_t st) {
+  size_t n = map_words > zero_words ? map_words : zero_words;
+  size_t blocks = (n + 255) / 256;
+  if (blocks < 1) blocks = 1;
+  if (blocks > 1024) blocks = 1024;
+  hipLaunchKernelGGL(fit_setup_single_kernel, dim3((unsigned)blocks), dim3(256), 0, st, d, d_desc, d_maps, map_words,
+                     zero_begin, zero_words);
+  return hipGetLastError();
+}
+
 // The adjoint launch of snapshot class c on stream st.
 template <int C>
 hipError_t launch_adjoint_class(const FitLaunch& L, int c, int bump, hipStream_t st) {

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/wt.h"
+#include "wt_fit.hpp"
 #include "wt_kernels.hpp"
 #include "wt_projector.hpp"
 
@@ -146,6 +147,10 @@ wt_status wt_create(wt_ctx** out) {
   wt_ctx* c = new (std::nothrow) wt_ctx();
   if (!c) return WT_ERR_OOM;
   c->device = dev;
+  // resolve this device's fixed fit scratch now, with the device current; a failure is
+  // kept and reported by wt_fit_loss_grad_static, the only entry point that uses it
+  std::string why;
+  (void)wt::static_fit_scratch(dev, &why);
   *out = c;
   return WT_OK;
 }

@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <string>
+
 #include "wt_device.hpp"
 
 namespace wt {
@@ -70,5 +72,18 @@ hipError_t launch_synthesize(const FitDesc& P, float* out, float sigma, unsigned
 
 // Zero n 32-bit words with a kernel (graph-capture safe).
 hipError_t launch_zero_words(uint32_t* p, size_t n, hipStream_t st);
+
+// Writes one problem's descriptor (passed by value) to d_desc, fills the block and row
+// maps (map_words ints at d_maps) with problem 0, and zeroes zero_words 32-bit words at
+// zero_begin, in one kernel: the set-up of a single-problem evaluation in memory the
+// caller did not allocate per call.
+hipError_t launch_fit_setup_single(const FitDesc& d, FitDesc* d_desc, int* d_maps, size_t map_words,
+                                   uint32_t* zero_begin, size_t zero_words, hipStream_t st);
+
+// The device address of this library's fixed single-problem fit scratch on `device`
+// (resolved once per device, with that device current), or null with a message.
+void* static_fit_scratch(int device, std::string* error);
+// Its size in bytes.
+size_t static_fit_scratch_bytes();
 
 }  // namespace wt

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -48,6 +49,77 @@ bool problem_ok(const wt_fit_problem& p, std::string* why) {
 }
 
 }  // namespace
+
+// ---------------------------------------------------------------- fixed scratch
+// Device memory of wt_fit_loss_grad_static: one problem's descriptor, its block and row
+// maps, Adam state it never steps, the loss, and the per-block partial sums, in a
+// file-scope device array, so the entry point never allocates. Each device holds its own
+// copy for the life of the process. Its address is resolved once per device with
+// hipGetSymbolAddress (the array's name in host code is only a stand-in for the symbol);
+// kernels receive that address.
+namespace {
+
+constexpr size_t kStaticFitScratchBytes = WT_FIT_STATIC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kStaticFitScratchBytes];
+
+std::mutex g_scratch_mutex;
+std::vector<void*> g_scratch_ptr;          // per device, once resolved
+std::vector<std::string> g_scratch_error;  // per device, when resolving failed
+std::vector<char> g_scratch_done;
+
+size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+// Layout of the scratch for a problem of `np` = 5 W parameters and `nblocks` adjoint
+// blocks: descriptor | maps (nblocks + np + 1 ints) | m, v (np floats each) | loss, step
+// | partials ((np + 1) x nblocks floats). Returns the total bytes; offsets through the
+// pointers.
+size_t static_fit_layout(int np, int nblocks, size_t* maps, size_t* state, size_t* partials) {
+  size_t off = align256(sizeof(wt::FitDesc));
+  *maps = off;
+  off = align256(off + sizeof(int) * ((size_t)nblocks + np + 1));
+  *state = off;
+  off = align256(off + sizeof(float) * (2 * (size_t)np + 2));
+  *partials = off;
+  return off + sizeof(float) * (size_t)(np + 1) * nblocks;
+}
+
+}  // namespace
+
+namespace wt {
+
+size_t static_fit_scratch_bytes() { return kStaticFitScratchBytes; }
+
+void* static_fit_scratch(int device, std::string* error) {
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (device < 0) {
+    *error = "invalid device";
+    return nullptr;
+  }
+  if ((size_t)device >= g_scratch_done.size()) {
+    g_scratch_ptr.resize(device + 1, nullptr);
+    g_scratch_error.resize(device + 1);
+    g_scratch_done.resize(device + 1, 0);
+  }
+  if (!g_scratch_done[device]) {
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e == hipSuccess && cur != device) e = hipSetDevice(device);
+    void* ptr = nullptr;
+    if (e == hipSuccess) e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e == hipSuccess && !ptr) e = hipErrorInvalidSymbol;
+    if (e == hipSuccess) {
+      g_scratch_ptr[device] = ptr;
+    } else {
+      g_scratch_error[device] = std::string("fixed fit scratch: ") + hipGetErrorString(e);
+    }
+    if (cur >= 0 && cur != device) (void)hipSetDevice(cur);
+    g_scratch_done[device] = 1;
+  }
+  if (!g_scratch_ptr[device]) *error = g_scratch_error[device];
+  return g_scratch_ptr[device];
+}
+
+}  // namespace wt
 
 extern "C" {
 
@@ -223,6 +295,70 @@ wt_status wt_fit_loss_grad(wt_fit* f, float* d_grad, float* d_losses, void* stre
   const wt::FitAdam a{0.0f, 0.0f, 0.0f, 0.0f};
   hipError_t e = wt::launch_fit_iteration(f->launch, a, 0, d_grad, d_losses, (hipStream_t)stream);
   return e == hipSuccess ? WT_OK : hip_err(f->ctx, e, "fit gradient launch");
+}
+
+wt_status wt_fit_loss_grad_static(wt_ctx* ctx, const wt_fit_problem* p, float* d_grad, float* d_loss, void* stream) {
+  if (!ctx || !p || !d_grad) return WT_ERR_INVALID_ARG;
+  std::string why;
+  if (!problem_ok(*p, &why)) return wt::ctx_fail(ctx, WT_ERR_INVALID_ARG, why);
+  const long long V = (long long)p->grid_side * p->grid_side;
+  const int cls = wt::fit_class((int)p->snapshots);
+  const long long per_block = (long long)wt::kFitThreads * wt::kFitPointsPerThread[cls];
+  const long long nblocks = (V + per_block - 1) / per_block;
+  const int np = (int)p->wave_count * 5;
+  size_t off_maps = 0, off_state = 0, off_partials = 0;
+  const size_t need = static_fit_layout(np, (int)std::min<long long>(nblocks, 1 << 30), &off_maps, &off_state,
+                                        &off_partials);
+  if (nblocks >= (1 << 30) || need > kStaticFitScratchBytes)
+    return wt::ctx_fail(ctx, WT_ERR_INVALID_ARG,
+                        "problem needs " + std::to_string(need) + " bytes of fixed scratch; the limit is " +
+                            std::to_string(kStaticFitScratchBytes) + " (fewer grid points or waves, or wt_fit_create)");
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_err(ctx, e, "wt_fit_loss_grad_static");
+  void* base = wt::static_fit_scratch(dev, &why);
+  if (!base) return wt::ctx_fail(ctx, WT_ERR_HIP, why);
+  char* b = static_cast<char*>(base);
+  wt::FitDesc d{};
+  d.spacing = p->spacing;
+  d.x0 = p->x0;
+  d.z0 = p->z0;
+  d.t0 = p->t0;
+  d.dt = p->dt;
+  d.g = p->gravity;
+  d.target = p->target;
+  d.params = p->params;
+  float* state = reinterpret_cast<float*>(b + off_state);
+  d.adam_m = state;
+  d.adam_v = state + np;
+  d.loss = state + 2 * np;
+  d.step = reinterpret_cast<int*>(state + 2 * np + 1);
+  d.partials = reinterpret_cast<float*>(b + off_partials);
+  d.n = (int)p->grid_side;
+  d.S = (int)p->snapshots;
+  d.W = (int)p->wave_count;
+  d.nblocks = (int)nblocks;
+  d.block_offset = 0;
+  d.row_offset = 0;
+  d.param_out_offset = 0;
+  const hipStream_t st = (hipStream_t)stream;
+  wt::FitDesc* d_desc = reinterpret_cast<wt::FitDesc*>(b);
+  int* d_maps = reinterpret_cast<int*>(b + off_maps);
+  // the previous call's contents stay in the array: rewrite the descriptor and maps and
+  // clear the state words; every partial is overwritten by the adjoint launch
+  e = wt::launch_fit_setup_single(d, d_desc, d_maps, (size_t)nblocks + np + 1, reinterpret_cast<uint32_t*>(state),
+                                  2 * (size_t)np + 2, st);
+  if (e != hipSuccess) return hip_err(ctx, e, "fixed-scratch set-up launch");
+  wt::FitLaunch L{};
+  L.probs = d_desc;
+  L.block_prob = d_maps;
+  L.row_prob = d_maps + nblocks;
+  L.class_begin[cls] = 0;
+  L.class_blocks[cls] = nblocks;
+  L.total_rows = np + 1;
+  const wt::FitAdam a{0.0f, 0.0f, 0.0f, 0.0f};
+  e = wt::launch_fit_iteration(L, a, 0, d_grad, d_loss, st);
+  return e == hipSuccess ? WT_OK : hip_err(ctx, e, "fit gradient launch");
 }
 
 wt_status wt_fit_synthesize(wt_ctx* ctx, const wt_fit_problem* p, float sigma, uint64_t seed, float* d_target,

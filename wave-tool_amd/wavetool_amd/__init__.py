@@ -413,6 +413,17 @@ def synthesize(ctx, problem, sigma, seed, target, stream=None):
                                        ctypes.c_void_p(target.data_ptr()), _stream(stream)))
 
 
+def fit_loss_grad_static(ctx, problem, grad, loss=None, stream=None):
+    """Loss and gradient of one problem through wt_fit_loss_grad_static (no wt_fit, no
+    allocation): grad a device float32 tensor of W*5 elements, loss None or one element."""
+    import torch
+    _check_array(grad, torch.float32, int(problem.wave_count) * 5, "grad")
+    if loss is not None:
+        _check_array(loss, torch.float32, 1, "loss")
+    ctx._check(lib().wt_fit_loss_grad_static(ctx.handle, ctypes.byref(problem), _ptr(grad), _ptr(loss),
+                                              _stream(stream)))
+
+
 def phase_plan(problem, budget_bytes=0):
     b = ctypes.c_uint64()
     r = ctypes.c_int()

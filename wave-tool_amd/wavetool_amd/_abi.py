@@ -31,6 +31,7 @@ WT_BYTES_PER_VERTEX_PNTB = 48
 WT_ADJOINT_FLOP_PER_VERTEX_WAVE_SNAPSHOT = 50
 WT_MAX_WAVES = 1024
 WT_MAX_VARYING_VERTICES = 2 * (2 ** 32 - 256)
+WT_FIT_STATIC_SCRATCH_BYTES = 16 << 20
 
 
 class GerstnerWave(ctypes.Structure):
@@ -219,6 +220,7 @@ PROTOTYPES = [
     ("wt_fit_reset", ctypes.c_int, [_vp, _vp]),
     ("wt_fit_iterate", ctypes.c_int, [_vp, ctypes.POINTER(AdamConfig), ctypes.c_uint32, _vp, _vp]),
     ("wt_fit_loss_grad", ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    ("wt_fit_loss_grad_static", ctypes.c_int, [_vp, ctypes.POINTER(FitProblem), _vp, _vp, _vp]),
     ("wt_fit_synthesize", ctypes.c_int,
      [_vp, ctypes.POINTER(FitProblem), ctypes.c_float, ctypes.c_uint64, _vp, _vp]),
     ("wt_fit_phase_plan", ctypes.c_int,
