@@ -2,13 +2,17 @@
 
 Headline workload (one step): the plain forward pass over the 1024 x 1024 projected
 grid with 64 Gerstner waves and the 512 x 512 R32F value-noise heightmap, 240 frames
-at t = 10 + k/60 s, captured as one HIP graph of 240 kernel launches that re-fill the
-same position / normal buffers (the per-frame vertex-buffer update of the
-reference's render loop). Inputs are resident in HBM before timing starts.
+at t = 10 + k/60 s (the per-frame vertex-buffer update of the reference's render loop).
+By default two frames are in flight: frame k runs on stream k mod 2 and writes
+vertex-buffer set k mod 2 (double-buffered vertex buffers), so one frame's epilogue and
+stores overlap the next frame's wave sums; --frames-in-flight 1 runs them one after
+another on one stream, and --loop graph replays the step as one captured HIP graph.
+Inputs are resident in HBM before timing starts.
 
 Prints one JSON line: vertex-waves/s (V * W * frames / wall time), the roofline of the
-forward kernel (algorithmic 26 FLOP per vertex-wave over the mean launch time from HIP
-events on the launch stream, against the 157.3 TFLOP/s FP32 peak), the CPU baseline
+forward kernel (algorithmic 26 FLOP per vertex-wave over the launch period from HIP
+events around the step's launches, against the 157.3 TFLOP/s FP32 peak, with each
+launch's own serial duration beside it), the CPU baseline
 (the oracle's FP32 instantiation over std::threads on a bounded frame sample of the
 same workload), and `extras` for the other workloads: the batched forward over 512
 uneven patches, the varyings pass, the fragment stage on a 3840 x 2160 frame, one
@@ -126,8 +130,11 @@ def bench_plain(torch, wt, scenes, args):
     waves = torch.from_numpy(waves_np).to(dev)
     hm = torch.from_numpy(texels).to(dev)
     V = N * N
-    pos = torch.empty(V * 3, dtype=torch.float32, device=dev)
-    nrm = torch.empty_like(pos)
+    nfl = max(1, args.frames_in_flight)
+    # one position / normal buffer set per frame in flight (double-buffered vertex
+    # buffers: frame k writes set k mod nfl)
+    bufs = [(torch.empty(V * 3, dtype=torch.float32, device=dev), torch.empty(V * 3, dtype=torch.float32, device=dev))
+            for _ in range(nfl)]
     ctx = wt.Context()
     if args.launch_shape:
         split, steps, lanes = (int(v) for v in args.launch_shape.split(","))
@@ -139,34 +146,70 @@ def bench_plain(torch, wt, scenes, args):
     shape = ctx.get_launch_config(N, W)
     params = [wt.water_params(corners, cam_pos, N, cfg["t0"] + k * cfg["dt"], cfg["sample_scale"],
                               cfg["displacement_scale"], 0.0, cfg["scroll"]) for k in range(F)]
-    stream = torch.cuda.Stream()
-    torch.cuda.synchronize()
-    ctx.graph_begin(stream)
-    for k in range(F):
-        ctx.displace(params[k], waves, pos, nrm, heightmap=hm, stream=stream)
-    graph = ctx.graph_end(stream)
+    streams = [torch.cuda.Stream() for _ in range(nfl)]
+    main = streams[0]
+
+    def frame_loop(n_in_flight):
+        """The step's F frames, frame k on stream k mod n_in_flight writing buffer set k mod
+        n_in_flight; the other streams fork from and join back to the first."""
+        ss = streams[:n_in_flight]
+        if n_in_flight > 1:
+            fork = torch.cuda.Event()
+            fork.record(main)
+            for s in ss[1:]:
+                s.wait_event(fork)
+        for k in range(F):
+            ctx.displace(params[k], waves, *bufs[k % n_in_flight], heightmap=hm, stream=ss[k % n_in_flight])
+        for s in ss[1:]:
+            join = torch.cuda.Event()
+            join.record(s)
+            main.wait_event(join)
+
+    graph = None
+    if args.loop == "graph":
+        torch.cuda.synchronize()
+        ctx.graph_begin(main)
+        frame_loop(nfl)
+        graph = ctx.graph_end(main)
+        step = lambda: graph.launch(main)  # noqa: E731
+    else:
+        step = lambda: frame_loop(nfl)  # noqa: E731
     for _ in range(args.warmup):
-        graph.launch(stream)
-    wall, ev = timed(torch, stream, lambda: graph.launch(stream), args.steps)
-    graph.close()
-    assert torch.isfinite(pos).all() and torch.isfinite(nrm).all()
+        step()
+    wall, ev = timed(torch, main, step, args.steps)
+    if graph is not None:
+        graph.close()
+    for P, Nn in bufs:
+        assert torch.isfinite(P).all() and torch.isfinite(Nn).all()
+    # the same frames one at a time on one stream: each launch's own duration
+    for _ in range(args.warmup):
+        frame_loop(1)
+    _, ev1 = timed(torch, main, lambda: frame_loop(1), args.steps)
     vw_per_step = V * W * F
     launch_s = ev / (args.steps * F)
+    serial_s = ev1 / (args.steps * F)
     flop_per_launch = wt.WT_FLOP_PER_VERTEX_WAVE * V * W
     achieved = flop_per_launch / launch_s / 1e12
     traffic, traffic_shape = load_traffic()
     roofline = dict(bound="valu", achieved=round(achieved, 3), peak=PEAK_FP32_TFLOPS, unit="TFLOP/s",
                     frac=round(achieved / PEAK_FP32_TFLOPS, 4), traffic=traffic, traffic_launch_shape=traffic_shape,
-                    launch_us=round(launch_s * 1e6, 3),
+                    launch_us=round(launch_s * 1e6, 3), frames_in_flight=nfl,
+                    serial_launch_us=round(serial_s * 1e6, 3),
+                    serial_frac=round(flop_per_launch / serial_s / 1e12 / PEAK_FP32_TFLOPS, 4),
                     hbm_write_GBs=round(wt.WT_BYTES_PER_VERTEX_PN * V / launch_s / 1e9, 1),
-                    note="achieved = 26 FLOP x V x W per launch / mean launch time (HIP events on the launch "
-                         "stream); FP32 VALU kernel, peak = MI355X FP32 157.3 TFLOP/s")
+                    note="achieved = 26 FLOP x V x W per launch / launch period (HIP events around the step's "
+                         "launches: elapsed / frames; with %d frames in flight a launch overlaps the next one's, "
+                         "so each launch's own duration, serial_launch_us, is measured beside it on one stream); "
+                         "FP32 VALU kernel, peak = MI355X FP32 157.3 TFLOP/s" % nfl)
     out = dict(metric=METRIC, value=vw_per_step * args.steps / wall, unit="vertex-waves/s", n_gpus=1,
                steps=args.steps, warmup=args.warmup, ms_per_step=wall * 1e3 / args.steps, higher_is_better=True,
                scaling="weak", vs_baseline=None, dtype="f32", data="synthetic",
                config=dict(workload="plain forward: 1024x1024 projected grid, 64 Gerstner waves, "
-                                    "512x512 R32F value-noise heightmap, 240 frames per step (one HIP graph)",
-                           grid=N, waves=W, frames_per_step=F, launch_shape=shape),
+                                    "512x512 R32F value-noise heightmap, 240 frames per step, %d frame(s) in flight "
+                                    "(frame k on stream k mod %d writing vertex-buffer set k mod %d), %s launches"
+                                    % (nfl, nfl, nfl, "graph-captured" if args.loop == "graph" else "eager"),
+                           grid=N, waves=W, frames_per_step=F, frames_in_flight=nfl, loop=args.loop,
+                           launch_shape=shape),
                roofline=roofline)
     ctx.close()
     return out, (corners, cam_pos, waves_np, texels, cfg)
@@ -354,6 +397,20 @@ def make_fits(torch, wt, scenes, ctx, specs, sigma=0.01):
     return problems, keep
 
 
+# loss of the hidden waves themselves: 1/2 of the mean squared noise over the three
+# displacement components, 1.5 sigma^2 at sigma = 0.01 m
+NOISE_FLOOR = 1.5 * 0.01 ** 2
+
+
+def fit_losses(torch, fit, specs, stream):
+    """Each problem's loss at its current parameters (one gradient evaluation)."""
+    g = torch.empty(sum(s["W"] * 5 for s in specs), device="cuda")
+    l = torch.empty(len(specs), device="cuda")
+    fit.loss_grad(g, l, stream=stream)
+    stream.synchronize()
+    return l.cpu().numpy().astype(np.float64)
+
+
 def bench_fit_single(torch, wt, scenes, iterations=500, cpu=True):
     ctx = wt.Context()
     spec = dict(n=512, S=16, W=32, seed=3)
@@ -361,17 +418,24 @@ def bench_fit_single(torch, wt, scenes, iterations=500, cpu=True):
     fit = wt.Fit(ctx, problems)
     losses = torch.zeros(1, device="cuda")
     stream = torch.cuda.Stream()
+    loss0 = fit_losses(torch, fit, [spec], stream)
+    init = keep[0][1].clone()
     torch.cuda.synchronize()
     ctx.graph_begin(stream)
+    fit.reset(stream=stream)
     fit.iterate(iterations, losses=losses, stream=stream, **ADAM)
     graph = ctx.graph_end(stream)
+    graph.launch(stream)  # warm-up replay, then the timed one from the same start
+    torch.cuda.synchronize()
+    keep[0][1].copy_(init)
     wall, _ = timed(torch, stream, lambda: graph.launch(stream))
     graph.close()
-    last = float(losses.item())
+    loss1 = fit_losses(torch, fit, [spec], stream)
     vws = spec["n"] ** 2 * spec["S"] * spec["W"]
     out = dict(workload="single fit: 512x512 grid at 0.5 m, 16 snapshots, 32 waves, %d Adam iterations "
                         "(one HIP graph)" % iterations,
-               value=1.0 / wall, unit="fits/s", ms_per_iteration=wall * 1e3 / iterations, loss_last=last,
+               value=1.0 / wall, unit="fits/s", ms_per_iteration=wall * 1e3 / iterations,
+               loss_initial=float(loss0[0]), loss_final=float(loss1[0]), noise_floor=NOISE_FLOOR,
                adjoint_algorithmic_TFLOPs=round(wt._abi.WT_ADJOINT_FLOP_PER_VERTEX_WAVE_SNAPSHOT * vws * iterations
                                                 / wall / 1e12, 2))
     if cpu:
@@ -397,37 +461,77 @@ def bench_fit_batched(torch, wt, scenes, iterations=300, count=256):
     problems, keep = make_fits(torch, wt, scenes, ctx, specs)
     init = [k[1].clone() for k in keep]
     stream = torch.cuda.Stream()
+
+    def restart():
+        torch.cuda.synchronize()
+        for k, p0 in zip(keep, init):
+            k[1].copy_(p0)
+        torch.cuda.synchronize()
+
     # batched: one problem table, the iterations captured in one graph
     fit = wt.Fit(ctx, problems)
+    loss0 = fit_losses(torch, fit, specs, stream)
     torch.cuda.synchronize()
     ctx.graph_begin(stream)
+    fit.reset(stream=stream)
     fit.iterate(iterations, stream=stream, **ADAM)
     graph = ctx.graph_end(stream)
+    graph.launch(stream)  # warm-up replay
+    restart()
     wall_b, _ = timed(torch, stream, lambda: graph.launch(stream))
     graph.close()
+    loss1 = fit_losses(torch, fit, specs, stream)
     batched_params = [k[1].clone() for k in keep]
     fit.close()
-    # plain: the same problems one by one, each its own single-problem fit, eager launches
-    for k, p0 in zip(keep, init):
-        k[1].copy_(p0)
+    # one by one: the same problems, each its own single-problem fit, each fit's iterations
+    # captured in its own graph (so the comparison is batching, not launch overhead),
+    # the graphs replayed in sequence
     singles = [wt.Fit(ctx, [p]) for p in problems]
+    graphs = []
     torch.cuda.synchronize()
+    for f in singles:
+        ctx.graph_begin(stream)
+        f.reset(stream=stream)
+        f.iterate(iterations, stream=stream, **ADAM)
+        graphs.append(ctx.graph_end(stream))
+    for g in graphs:  # warm-up replay
+        g.launch(stream)
+    restart()
 
     def run_plain():
-        for f in singles:
-            f.iterate(iterations, stream=stream, **ADAM)
+        for g in graphs:
+            g.launch(stream)
     wall_p, _ = timed(torch, stream, run_plain)
     same = all(torch.equal(k[1], b) for k, b in zip(keep, batched_params))
+    for g in graphs:
+        g.close()
+    # the same one-by-one fits with eager launches (what a caller without graphs sees)
+    restart()
+
+    def run_eager():
+        for f in singles:
+            f.reset(stream=stream)
+            f.iterate(iterations, stream=stream, **ADAM)
+    wall_e, _ = timed(torch, stream, run_eager)
     for f in singles:
         f.close()
     vws = sum(s["n"] ** 2 * s["S"] * s["W"] for s in specs)
     ctx.close()
+    near = loss1 <= 2.0 * NOISE_FLOOR
     return dict(workload="batched fits: %d fits (side 64-512 at 0.5 m, 8-64 waves, 4-32 snapshots), %d Adam iterations; "
                          "batched = one problem table, one HIP graph" % (count, iterations),
                 value=count / wall_b, unit="fits/s",
                 plain_one_by_one=dict(value=count / wall_p, unit="fits/s",
-                                      note="each problem its own single-problem fit, eager launches, in sequence"),
-                speedup_vs_plain=round(wall_p / wall_b, 2), results_identical=bool(same),
+                                      note="each problem its own single-problem fit, its iterations captured in its "
+                                           "own HIP graph, the graphs replayed in sequence"),
+                plain_one_by_one_eager=dict(value=count / wall_e, unit="fits/s",
+                                            note="the same, eager launches"),
+                speedup_vs_plain=round(wall_p / wall_b, 2), speedup_vs_eager=round(wall_e / wall_b, 2),
+                results_identical=bool(same),
+                loss_initial_median=float(np.median(loss0)), loss_final_median=float(np.median(loss1)),
+                loss_final_max=float(loss1.max()), noise_floor=NOISE_FLOOR,
+                share_within_2x_noise_floor=round(float(near.mean()), 4),
+                share_loss_reduced_10x=round(float((loss1 <= 0.1 * loss0).mean()), 4),
                 vertex_wave_snapshots_per_iteration=vws)
 
 
@@ -440,6 +544,10 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--launch-shape", default="", help="SPLIT,STEPS,LANES: fixed forward launch shape, no autotune")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--frames-in-flight", type=int, default=2,
+                    help="frames of the step whose launches may overlap (one stream and buffer set each)")
+    ap.add_argument("--loop", choices=("eager", "graph"), default="eager",
+                    help="the step's launches issued directly, or captured once as one HIP graph and replayed")
     ap.add_argument("--shade-only", action="store_true",
                     help="run only the fragment-stage workload and print its line (for profiling)")
     args = ap.parse_args()
