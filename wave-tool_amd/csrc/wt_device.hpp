@@ -98,13 +98,34 @@ struct HmTap {
   float t00, t10, t01, t11, a, b;
 };
 
+// Texel index wrap for the two taps of a footprint (GL REPEAT / MIRRORED_REPEAT over
+// n texels; one period is n, or 2n for the mirror). wrap_lo takes i in [-1, period - 1]
+// (only -1 is out of range), wrap_hi takes i in [0, period] (only period is): an unsigned
+// min picks the in-range one of i and i -/+ period. The mirror then folds [n, 2n) back
+// onto [0, n) as 2n - 1 - i.
 template <int M>
-__device__ __forceinline__ int wrap_index(int i, int n) {
-  // i in [-1, period]: one period of REPEAT (n) or MIRRORED_REPEAT (2n) folded onto [0, n)
-  const int period = tex_mirror(M) ? 2 * n : n;
-  i = i < 0 ? i + period : (i >= period ? i - period : i);
-  if (tex_mirror(M)) i = i < n ? i : (2 * n - 1 - i);
-  return i;
+__device__ __forceinline__ uint32_t wrap_fold(uint32_t i, uint32_t n) {
+  return tex_mirror(M) ? min(i, 2u * n - 1u - i) : i;
+}
+template <int M>
+__device__ __forceinline__ uint32_t wrap_lo(int i, uint32_t n) {
+  const uint32_t period = tex_mirror(M) ? 2u * n : n;
+  return wrap_fold<M>(min((uint32_t)i, (uint32_t)i + period), n);
+}
+template <int M>
+__device__ __forceinline__ uint32_t wrap_hi(int i, uint32_t n) {
+  const uint32_t period = tex_mirror(M) ? 2u * n : n;
+  return wrap_fold<M>(min((uint32_t)i, (uint32_t)i - period), n);
+}
+
+// One texel at a 32-bit element offset from the (wave-uniform) texel base: the address
+// is the base plus a zero-extended 32-bit byte offset (a heightmap holds at most 2^28
+// texels, 1 GiB in R32F), so no 64-bit address arithmetic per tap.
+template <int M>
+__device__ __forceinline__ float load_texel(const void* __restrict__ texels, uint32_t idx) {
+  const char* base = (const char*)texels;
+  if (tex_r8(M)) return (float)*(const uint8_t*)(base + idx);
+  return *(const float*)(base + (idx << 2));
 }
 
 template <int M>
@@ -119,21 +140,14 @@ __device__ __forceinline__ HmTap heightmap_fetch(const void* __restrict__ texels
   // fmaxf/fminf return the non-NaN operand, so a NaN coordinate lands on texel -1
   const int i0 = (int)fminf(fmaxf(fu, -1.0f), (float)(period_w - 1));
   const int j0 = (int)fminf(fmaxf(fv, -1.0f), (float)(period_h - 1));
-  const int c0 = wrap_index<M>(i0, W), c1 = wrap_index<M>(i0 + 1, W);
-  const int r0 = wrap_index<M>(j0, H) * W, r1 = wrap_index<M>(j0 + 1, H) * W;
-  if (tex_r8(M)) {
-    const uint8_t* p = (const uint8_t*)texels;
-    h.t00 = (float)p[r0 + c0];
-    h.t10 = (float)p[r0 + c1];
-    h.t01 = (float)p[r1 + c0];
-    h.t11 = (float)p[r1 + c1];
-  } else {
-    const float* p = (const float*)texels;
-    h.t00 = p[r0 + c0];
-    h.t10 = p[r0 + c1];
-    h.t01 = p[r1 + c0];
-    h.t11 = p[r1 + c1];
-  }
+  const uint32_t c0 = wrap_lo<M>(i0, (uint32_t)W), c1 = wrap_hi<M>(i0 + 1, (uint32_t)W);
+  // rows times the width in 24-bit multiplies (both factors below 2^14)
+  const uint32_t r0 = __umul24(wrap_lo<M>(j0, (uint32_t)H), (uint32_t)W);
+  const uint32_t r1 = __umul24(wrap_hi<M>(j0 + 1, (uint32_t)H), (uint32_t)W);
+  h.t00 = load_texel<M>(texels, r0 + c0);
+  h.t10 = load_texel<M>(texels, r0 + c1);
+  h.t01 = load_texel<M>(texels, r1 + c0);
+  h.t11 = load_texel<M>(texels, r1 + c1);
   return h;
 }
 
@@ -194,9 +208,25 @@ __device__ __forceinline__ VertexPending vertex_fetch(const EpilogueArgs& e, dou
   return v;
 }
 
+// The output rows of one grid row: the four arrays advanced to the row's first vertex
+// (64-bit arithmetic once per row; each vertex then adds a 32-bit column offset).
+struct RowOut {
+  float* pos;
+  float* nrm;
+  float* tan;
+  float* bit;
+};
+__device__ __forceinline__ float* row_at(float* row, uint32_t byte_offset) {
+  return (float*)((char*)row + byte_offset);
+}
+__device__ __forceinline__ RowOut row_out(const EpilogueArgs& e, size_t row_first_vertex) {
+  const size_t o = 3 * row_first_vertex;
+  return RowOut{e.pos + o, e.nrm + o, e.tan ? e.tan + o : nullptr, e.bit ? e.bit + o : nullptr};
+}
+
 template <int M>
 __device__ __forceinline__ void vertex_finish(const EpilogueArgs& e, const VertexPending& v, const float acc[8],
-                                              size_t vid) {
+                                              const RowOut& ro, uint32_t col) {
   const float Cx = acc[2], Cz = acc[3];
   const float Y = acc[4], Sxx = acc[5], Sxz = acc[6], Szz = acc[7];
   const float px = v.px, pz = v.pz;
@@ -224,15 +254,18 @@ __device__ __forceinline__ void vertex_finish(const EpilogueArgs& e, const Verte
   if (nx * vx + ny * vy + nz * vz < 0.0f) { nx = -nx; ny = -ny; nz = -nz; }
   // the outputs are written once per frame and not read back by this kernel: stream
   // them past the caches (4% faster on the 1024^2 x 64-wave frame)
-  store3_nt(e.pos + 3 * vid, px, py, pz);
-  store3_nt(e.nrm + 3 * vid, nx, ny, nz);
-  if (e.tan) {
+  // byte offset within the row as a 32-bit value (a row is at most 32768 x 12 bytes), so
+  // each store adds it to the row pointer without 64-bit shifts
+  const uint32_t b = 12u * col;
+  store3_nt(row_at(ro.pos, b), px, py, pz);
+  store3_nt(row_at(ro.nrm, b), nx, ny, nz);
+  if (ro.tan) {
     const float it = __builtin_amdgcn_rsqf(tx * tx + ty * ty + tz * tz);
-    store3_nt(e.tan + 3 * vid, tx * it, ty * it, tz * it);
+    store3_nt(row_at(ro.tan, b), tx * it, ty * it, tz * it);
   }
-  if (e.bit) {
+  if (ro.bit) {
     const float ib = __builtin_amdgcn_rsqf(bx * bx + by * by + bz * bz);
-    store3_nt(e.bit + 3 * vid, bx * ib, by * ib, bz * ib);
+    store3_nt(row_at(ro.bit, b), bx * ib, by * ib, bz * ib);
   }
 }
 

@@ -73,11 +73,14 @@ struct StepEntry {
 // The three-term (Chebyshev) recurrence x[j+1] = 2 cos(d) x[j] - x[j-1] generates the
 // cos / sin sequence with one FMA per value instead of the rotation's two, but an FP32
 // rounding e of 2 cos(d) shifts the step by e / (2 |sin d|) and local rounding errors
-// grow by up to 1 / |sin d|; with |sin d| >= 0.3 and K <= 16 steps both stay below
-// 2e-6 rad / 6e-6 relative. A wave takes the recurrence for a segment when every row of
-// the segment satisfies this, i.e. when the step d = 2 pi f has f mod 1/2 in
-// [kChebMinRev, 1/2 - kChebMinRev], kChebMinRev = asin(0.3) / (2 pi).
-constexpr double kChebMinRev = 0.048493;
+// grow by up to min(j, 1 / |sin d|) after j steps. Simulated in FP32 from FP32 seeds
+// over the K <= 16 steps of a lane, the worst absolute error of a unit cos / sin stays
+// at 8e-6 for |sin d| >= 0.1 (2.6e-6 at 0.3, 1e-5 at 0.02: past 1 / K the linear bound
+// takes over), i.e. 8e-6 A of a wave's displacement. A wave takes the recurrence for a
+// segment when every row of the segment satisfies |sin d| >= 0.1, i.e. when the step
+// d = 2 pi f has f mod 1/2 in [kChebMinRev, 1/2 - kChebMinRev], kChebMinRev =
+// asin(0.1) / (2 pi); the rest rotate.
+constexpr double kChebMinRev = 0.015942;
 
 // Build the step table of the waves this wavefront sums (i = sid, sid + S, ...) for the
 // LR rows of the segment starting at column col_base, in FP64, rounded once. Waves that
@@ -209,11 +212,11 @@ __device__ __forceinline__ void epilogue_steps(const FwdParams& p, const RowGeom
     const double x = fma((double)col, g.xs, g.xa), z = fma((double)col, g.zs, g.za);
     pend[u] = vertex_fetch<kTex>(p.ep, x, z, acc[JB + u]);
   }
+  const RowOut ro = row_out(p.ep, (size_t)(row < p.N ? row : 0) * (size_t)p.N);
 #pragma unroll
   for (int u = 0; u < KS; ++u) {
     const int col = col0 + (JB + u) * LC;
-    if (active && row < p.N && col < p.N)
-      vertex_finish<kTex>(p.ep, pend[u], acc[JB + u], (size_t)row * (size_t)p.N + (size_t)col);
+    if (active && row < p.N && col < p.N) vertex_finish<kTex>(p.ep, pend[u], acc[JB + u], ro, (uint32_t)col);
   }
 }
 
