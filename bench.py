@@ -137,12 +137,13 @@ def bench_plain(torch, wt, scenes, args):
             for _ in range(nfl)]
     ctx = wt.Context()
     if args.launch_shape:
-        split, steps, lanes = (int(v) for v in args.launch_shape.split(","))
-        ctx.set_launch_config(split, steps, lanes, 64)
-        log("launch shape forced to split %d, steps %d, lanes per row %d" % (split, steps, lanes))
+        v = [int(x) for x in args.launch_shape.split(",")]
+        chunk = v[3] if len(v) > 3 else 64
+        ctx.set_launch_config(v[0], v[1], v[2], chunk)
+        log("launch shape forced to split %d, steps %d, lanes per row %d, wave chunk %d" % (v[0], v[1], v[2], chunk))
     elif not args.no_autotune:
-        shape, ms = ctx.autotune(N, W, True)
-        log("autotuned launch shape %s (%.3f ms per frame)" % (shape, ms))
+        shape, ms = ctx.autotune(N, W, True, frames_in_flight=nfl)
+        log("autotuned launch shape %s (%.3f ms per frame, %d in flight)" % (shape, ms, nfl))
     shape = ctx.get_launch_config(N, W)
     params = [wt.water_params(corners, cam_pos, N, cfg["t0"] + k * cfg["dt"], cfg["sample_scale"],
                               cfg["displacement_scale"], 0.0, cfg["scroll"]) for k in range(F)]
@@ -216,7 +217,7 @@ def bench_plain(torch, wt, scenes, args):
 
 
 # --------------------------------------------------------------------- batched forward, 512 patches
-def bench_batched_forward(torch, wt, scenes, frames=120):
+def bench_batched_forward(torch, wt, scenes, frames=120, one_by_one=True):
     dev = torch.device("cuda:0")
     specs = scenes.batched_patches(seed=2, count=512)
     patches, waves_np, V = wt.batch_inputs(specs)
@@ -253,6 +254,12 @@ def bench_batched_forward(torch, wt, scenes, frames=120):
     jgraph.launch(stream)
     jwall, _ = timed(torch, stream, lambda: jgraph.launch(stream))
     jgraph.close()
+    if not one_by_one:
+        batch.close()
+        ctx.close()
+        return dict(workload="batched forward: 512 patches, %d frames in one HIP graph (device projector and one "
+                             "launch per segment shape per frame)" % frames,
+                    value=vw_frame * frames / wall, unit="vertex-waves/s", ms_per_frame=wall * 1e3 / frames)
     # the same frame as one plain launch per patch (no graph), for the batching gain
     params = [wt.water_params(wt.np_corners(p), s["camera"][4:7], s["N"], s["time"], 0.02, 0.05, 0.0, (0.3, 0.0))
               for s, p in zip(specs, patches)]
@@ -455,7 +462,7 @@ def bench_fit_single(torch, wt, scenes, iterations=500, cpu=True):
     return out
 
 
-def bench_fit_batched(torch, wt, scenes, iterations=300, count=256):
+def bench_fit_batched(torch, wt, scenes, iterations=300, count=256, one_by_one=True):
     ctx = wt.Context()
     specs = scenes.fit_batch_specs(seed=4, count=count)
     problems, keep = make_fits(torch, wt, scenes, ctx, specs)
@@ -483,6 +490,10 @@ def bench_fit_batched(torch, wt, scenes, iterations=300, count=256):
     loss1 = fit_losses(torch, fit, specs, stream)
     batched_params = [k[1].clone() for k in keep]
     fit.close()
+    if not one_by_one:
+        ctx.close()
+        return dict(workload="batched fits: %d fits, %d Adam iterations, one HIP graph" % (count, iterations),
+                    value=count / wall_b, unit="fits/s", loss_final_median=float(np.median(loss1)))
     # one by one: the same problems, each its own single-problem fit, each fit's iterations
     # captured in its own graph (so the comparison is batching, not launch overhead),
     # the graphs replayed in sequence
@@ -535,6 +546,28 @@ def bench_fit_batched(torch, wt, scenes, iterations=300, count=256):
                 vertex_wave_snapshots_per_iteration=vws)
 
 
+def run_extra(name, torch, wt, scenes, args):
+    """One of the secondary workloads, with a one-line log of its figure."""
+    cpu = not args.no_cpu_baseline
+    if name == "batched_forward":
+        r = bench_batched_forward(torch, wt, scenes, frames=args.frames, one_by_one=not args.no_one_by_one)
+        log("batched forward %.4g vertex-waves/s" % r["value"])
+    elif name == "varyings":
+        r = bench_varyings(torch, wt, scenes)
+        log("varyings %.4g vertices/s, %.1f GB/s" % (r["value"], r["roofline"]["achieved"]))
+    elif name == "shade":
+        r = bench_shade(torch, wt, scenes, cpu=cpu)
+        log("fragment stage %.4g fragments/s, %.1f GB/s" % (r["value"], r["roofline"]["achieved"]))
+    elif name == "fit_single":
+        r = bench_fit_single(torch, wt, scenes, iterations=args.fit_iterations or 500, cpu=cpu)
+        log("single fit %.4g fits/s" % r["value"])
+    else:
+        r = bench_fit_batched(torch, wt, scenes, iterations=args.fit_iterations or 300,
+                              one_by_one=not args.no_one_by_one)
+        log("batched fits %.4g fits/s (%sx one by one)" % (r["value"], r.get("speedup_vs_plain")))
+    return r
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -542,7 +575,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--launch-shape", default="", help="SPLIT,STEPS,LANES: fixed forward launch shape, no autotune")
+    ap.add_argument("--launch-shape", default="",
+                    help="SPLIT,STEPS,LANES[,CHUNK]: fixed forward launch shape, no autotune")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--frames-in-flight", type=int, default=2,
                     help="frames of the step whose launches may overlap (one stream and buffer set each)")
@@ -550,6 +584,13 @@ def main():
                     help="the step's launches issued directly, or captured once as one HIP graph and replayed")
     ap.add_argument("--shade-only", action="store_true",
                     help="run only the fragment-stage workload and print its line (for profiling)")
+    ap.add_argument("--only", choices=("batched_forward", "varyings", "shade", "fit_single", "fit_batched"),
+                    default="", help="run only this secondary workload and print its line (for profiling)")
+    ap.add_argument("--frames", type=int, default=120, help="frames of the batched-forward workload")
+    ap.add_argument("--fit-iterations", type=int, default=0,
+                    help="Adam iterations of the fit workloads (default: 500 single, 300 batched)")
+    ap.add_argument("--no-one-by-one", action="store_true",
+                    help="skip the one-by-one comparators of the batched workloads (for counter passes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (requested --gpus %d)" % args.gpus,
@@ -563,7 +604,9 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     if args.shade_only:
-        print(json.dumps(bench_shade(torch, wt, scenes, cpu=not args.no_cpu_baseline)), flush=True)
+        args.only = "shade"
+    if args.only:
+        print(json.dumps(run_extra(args.only, torch, wt, scenes, args)), flush=True)
         return
     out, inputs = bench_plain(torch, wt, scenes, args)
     log("plain forward %.4g vertex-waves/s, %.2f us per frame" % (out["value"], out["roofline"]["launch_us"]))
@@ -571,19 +614,8 @@ def main():
         out["cpu_baseline"] = cpu_baseline(*inputs)
     if not args.no_extras:
         extras = {}
-        extras["batched_forward"] = bench_batched_forward(torch, wt, scenes)
-        log("batched forward %.4g vertex-waves/s" % extras["batched_forward"]["value"])
-        extras["varyings"] = bench_varyings(torch, wt, scenes)
-        log("varyings %.4g vertices/s, %.1f GB/s" % (extras["varyings"]["value"],
-                                                      extras["varyings"]["roofline"]["achieved"]))
-        extras["shade"] = bench_shade(torch, wt, scenes, cpu=not args.no_cpu_baseline)
-        log("fragment stage %.4g fragments/s, %.1f GB/s" % (extras["shade"]["value"],
-                                                            extras["shade"]["roofline"]["achieved"]))
-        extras["fit_single"] = bench_fit_single(torch, wt, scenes, cpu=not args.no_cpu_baseline)
-        log("single fit %.4g fits/s" % extras["fit_single"]["value"])
-        extras["fit_batched"] = bench_fit_batched(torch, wt, scenes)
-        log("batched fits %.4g fits/s (%.2fx one by one)" % (extras["fit_batched"]["value"],
-                                                         extras["fit_batched"]["speedup_vs_plain"]))
+        for name in ("batched_forward", "varyings", "shade", "fit_single", "fit_batched"):
+            extras[name] = run_extra(name, torch, wt, scenes, args)
         out["extras"] = extras
     print(json.dumps(out), flush=True)
 

@@ -139,7 +139,7 @@ typedef struct wt_launch_config {
   uint32_t wave_split;     /* wavefronts sharing one block of vertices, each summing 1/wave_split of the waves: 1 or 2 */
   uint32_t steps_per_lane; /* vertices each lane advances along its row by the sin/cos recurrence: 4, 8 or 16 */
   uint32_t lanes_per_row;  /* lanes of a 64-wide wavefront sharing one grid row: 64, 32, 16 or 8 */
-  uint32_t wave_chunk;     /* waves staged in LDS per pass (64 in this build) */
+  uint32_t wave_chunk;     /* waves staged in LDS per pass: 64, or 32 / 128 with wave_split 2 and 64 lanes per row */
 } wt_launch_config;
 
 /* ---------------------------------------------------------------- context */
@@ -226,12 +226,19 @@ wt_status wt_set_launch_config(wt_ctx* ctx, const wt_launch_config* cfg);
 /* The launch shape wt_water_displace would use for this problem size. */
 wt_status wt_get_launch_config(wt_ctx* ctx, uint32_t grid_length, uint32_t wave_count, wt_launch_config* cfg);
 
-/* Times every supported launch shape for (grid_length, wave_count) on the
+/* Times every supported launch shape (steps per lane, lanes per row and wave split,
+ * i.e. vertices per workgroup, and the wave chunk) for (grid_length, wave_count) on the
  * context's device with synthetic inputs and stores the fastest in the
  * context's table (per size bucket). best_ms (may be NULL) receives its
  * per-launch time. Allocates and frees its own scratch buffers. */
 wt_status wt_autotune_forward(wt_ctx* ctx, uint32_t grid_length, uint32_t wave_count, int with_heightmap,
                               wt_launch_config* best, float* best_ms);
+/* wt_autotune_forward timing each shape with frames_in_flight (1..4) launches in flight
+ * (frame r on stream r mod frames_in_flight, writing its own output buffers), as a
+ * renderer with that many frames in flight runs it; best_ms is the time per frame.
+ * Stores the winner in the same table. No reference counterpart. */
+wt_status wt_autotune_forward_overlapped(wt_ctx* ctx, uint32_t grid_length, uint32_t wave_count, int with_heightmap,
+                                         uint32_t frames_in_flight, wt_launch_config* best, float* best_ms);
 /* The batched kernels' segment shape for patches of side ~grid_length with
  * ~wave_count waves: times each of the WT_BATCH_SHAPES shapes on a synthetic batch of
  * such patches (about 4 M vertices) and records the fastest for that (N, W) size

@@ -82,3 +82,25 @@ def test_tuned_batch_matches_plain_launches(torch_cuda, wtctx):
         assert torch.equal(P, pos[lo:hi]) and torch.equal(Nn, nrm[lo:hi]), i
     batch.close()
     ctx.close()
+
+
+def test_forward_autotune_overlapped_and_chunks(torch_cuda):
+    """The overlapped autotuner (two and three frames in flight) returns a supported shape
+    and a per-frame time, stores it in the table, and a 32 / 128 wave chunk is accepted only
+    on the one-row shapes (two wavefronts per segment, 64 lanes per row)."""
+    ctx = wt.Context()
+    say("autotune overlapped N=256 W=40")
+    for nfl in (2, 3):
+        cfg, ms = ctx.autotune(256, 40, True, frames_in_flight=nfl)
+        assert ms > 0.0 and cfg["wave_chunk"] in (32, 64, 128)
+        assert ctx.get_launch_config(256, 40) == cfg
+        if cfg["wave_chunk"] != 64:
+            assert cfg["wave_split"] == 2 and cfg["lanes_per_row"] == 64
+    with pytest.raises(wt.WtError):
+        ctx.autotune(256, 40, True, frames_in_flight=5)
+    ctx.set_launch_config(2, 8, 64, 128)
+    ctx.set_launch_config(2, 16, 64, 32)
+    for bad in ((2, 8, 32, 128), (1, 8, 64, 32), (2, 8, 64, 96)):
+        with pytest.raises(wt.WtError):
+            ctx.set_launch_config(*bad)
+    ctx.close()

@@ -99,7 +99,7 @@ def test_grid1024_w64_heightmap_frame_parity(torch_cuda, wtctx, frame, shape):
         wtctx.set_launch_config(*BENCH_SHAPE, 64)
     elif shape == "autotuned":
         ctx = wt.Context()
-        ctx.autotune(cfg["N"], cfg["W"], True)
+        ctx.autotune(cfg["N"], cfg["W"], True, frames_in_flight=2)
     say("1024x1024 grid, W=64, heightmap, frame %d, %s shape %s" % (frame, shape,
                                                                      ctx.get_launch_config(cfg["N"], cfg["W"])))
     try:
@@ -116,20 +116,21 @@ def test_grid1024_w64_heightmap_frame_parity(torch_cuda, wtctx, frame, shape):
     assert res["n_flip"] < 1e-4 * res["n_near"]
 
 
-SHAPES = [(v, k, lc) for v in (1, 2) for k in (4, 8, 16) for lc in (64, 32, 16, 8)]
+SHAPES = ([(v, k, lc, 64) for v in (1, 2) for k in (4, 8, 16) for lc in (64, 32, 16, 8)] +
+          [(2, k, 64, ch) for k in (4, 8, 16) for ch in (32, 128)])
 
 
-@pytest.mark.parametrize("split,steps,lanes", SHAPES)
-def test_launch_shapes(torch_cuda, wtctx, split, steps, lanes):
-    """Every launch shape on a ragged size (N = 301 is no multiple of any segment
-    width) and a wave count that spans two LDS chunks (W = 70)."""
-    say("shape split=%d steps=%d lanes=%d N=301 W=70" % (split, steps, lanes))
+@pytest.mark.parametrize("split,steps,lanes,chunk", SHAPES)
+def test_launch_shapes(torch_cuda, wtctx, split, steps, lanes, chunk):
+    """Every launch shape and wave chunk on a ragged size (N = 301 is no multiple of any
+    segment width) and a wave count that spans two or three LDS chunks (W = 70)."""
+    say("shape split=%d steps=%d lanes=%d chunk=%d N=301 W=70" % (split, steps, lanes, chunk))
     cam = scenes.camera_tuple(height=6.0, pitch_deg=-25.0, yaw_deg=30.0)
     corners = wt.projector_corners(cam, 2.0)
     waves = scenes.gen_waves(11, 70)
     texels = scenes.value_noise(5, 64)
     cfg = dict(N=301, sample_scale=0.05, displacement_scale=0.1, scroll=(0.2, -0.1))
-    wtctx.set_launch_config(split, steps, lanes, 64)
+    wtctx.set_launch_config(split, steps, lanes, chunk)
     try:
         res = check_parity(torch_cuda, wtctx, cfg, corners, waves, cam[4:7], 37.25, texels, True)
     finally:

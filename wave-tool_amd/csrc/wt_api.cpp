@@ -51,11 +51,17 @@ int bucket_log2(uint32_t v) {
   return b;
 }
 
+// Wave chunks a launch shape can stage: 32, 64 or 128 waves for the one-row shapes
+// (two wavefronts per segment, 64 lanes per row), 64 for the others.
+bool chunk_ok(const wt::LaunchShape& s) {
+  if (s.wave_chunk == 64 || s.wave_chunk == 0) return true;
+  return (s.wave_chunk == 32 || s.wave_chunk == 128) && s.split == 2 && s.lanes_per_row == 64;
+}
+
 bool valid_shape(const wt::LaunchShape& s) {
   const bool k_ok = s.steps == 4 || s.steps == 8 || s.steps == 16;
   const bool lc_ok = s.lanes_per_row == 64 || s.lanes_per_row == 32 || s.lanes_per_row == 16 || s.lanes_per_row == 8;
-  return (s.split == 1 || s.split == 2) && k_ok && lc_ok && (s.wave_chunk == 64 || s.wave_chunk == 0) &&
-         s.steps >= 2 * (s.split - 1);
+  return (s.split == 1 || s.split == 2) && k_ok && lc_ok && chunk_ok(s) && s.steps >= 2 * (s.split - 1);
 }
 
 wt::LaunchShape default_shape(uint32_t N) {
@@ -341,15 +347,21 @@ wt_status wt_get_launch_config(wt_ctx* ctx, uint32_t N, uint32_t W, wt_launch_co
   return WT_OK;
 }
 
-wt_status wt_autotune_forward(wt_ctx* ctx, uint32_t N, uint32_t W, int with_hm, wt_launch_config* best,
-                              float* best_ms) {
+// Times every supported launch shape (and, for the one-row shapes, every wave chunk) on
+// synthetic inputs of this size with `nfl` frames in flight (frame r on stream r mod nfl,
+// writing buffer set r mod nfl) and stores the fastest per frame in the context's table.
+static wt_status autotune_forward(wt_ctx* ctx, uint32_t N, uint32_t W, int with_hm, uint32_t nfl,
+                                  wt_launch_config* best, float* best_ms) {
   if (!ctx) return WT_ERR_INVALID_ARG;
   if (N < 2 || N > 8192 || W > WT_MAX_WAVES) return fail(ctx, WT_ERR_INVALID_ARG, "autotune size out of range");
+  if (nfl < 1 || nfl > 4) return fail(ctx, WT_ERR_INVALID_ARG, "frames_in_flight must be in [1, 4]");
   const size_t V = (size_t)N * N;
-  float *pos = nullptr, *nrm = nullptr, *tex = nullptr;
+  float* pos[4] = {};
+  float* nrm[4] = {};
+  float* tex = nullptr;
   wt_gerstner_wave* dw = nullptr;
-  hipStream_t st = nullptr;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipStream_t st[4] = {};
+  hipEvent_t e0 = nullptr, e1 = nullptr, join[4] = {};
   wt_status rs = WT_OK;
   const uint32_t TW = 256;
   std::vector<wt_gerstner_wave> hw(W ? W : 1);
@@ -362,13 +374,16 @@ wt_status wt_autotune_forward(wt_ctx* ctx, uint32_t N, uint32_t W, int with_hm, 
   std::vector<float> htex(TW * TW);
   for (uint32_t i = 0; i < TW * TW; ++i) htex[i] = 0.5f + 0.5f * std::sin(0.01f * (float)i);
   auto cleanup = [&]() {
-    if (pos) (void)hipFree(pos);
-    if (nrm) (void)hipFree(nrm);
+    for (uint32_t j = 0; j < 4; ++j) {
+      if (pos[j]) (void)hipFree(pos[j]);
+      if (nrm[j]) (void)hipFree(nrm[j]);
+      if (st[j]) (void)hipStreamDestroy(st[j]);
+      if (join[j]) (void)hipEventDestroy(join[j]);
+    }
     if (tex) (void)hipFree(tex);
     if (dw) (void)hipFree(dw);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
-    if (st) (void)hipStreamDestroy(st);
   };
 #define WT_TRY(x)                                \
   do {                                           \
@@ -379,13 +394,16 @@ wt_status wt_autotune_forward(wt_ctx* ctx, uint32_t N, uint32_t W, int with_hm, 
       return rs;                                 \
     }                                            \
   } while (0)
-  WT_TRY(hipMalloc(&pos, V * 12));
-  WT_TRY(hipMalloc(&nrm, V * 12));
+  for (uint32_t j = 0; j < nfl; ++j) {
+    WT_TRY(hipMalloc(&pos[j], V * 12));
+    WT_TRY(hipMalloc(&nrm[j], V * 12));
+    WT_TRY(hipStreamCreateWithFlags(&st[j], hipStreamNonBlocking));
+    WT_TRY(hipEventCreateWithFlags(&join[j], hipEventDisableTiming));
+  }
   WT_TRY(hipMalloc(&dw, sizeof(wt_gerstner_wave) * hw.size()));
   WT_TRY(hipMalloc(&tex, sizeof(float) * htex.size()));
   WT_TRY(hipMemcpy(dw, hw.data(), sizeof(wt_gerstner_wave) * hw.size(), hipMemcpyHostToDevice));
   WT_TRY(hipMemcpy(tex, htex.data(), sizeof(float) * htex.size(), hipMemcpyHostToDevice));
-  WT_TRY(hipStreamCreate(&st));
   WT_TRY(hipEventCreate(&e0));
   WT_TRY(hipEventCreate(&e1));
   wt_water_params p{};
@@ -397,42 +415,67 @@ wt_status wt_autotune_forward(wt_ctx* ctx, uint32_t N, uint32_t W, int with_hm, 
   p.heightmapDisplacementScale = 0.05f;
   p.waveAnimationTimeInSeconds = 10.0;
   wt_heightmap hm{tex, TW, TW, WT_TEXEL_R32F, WT_WRAP_REPEAT};
-  wt_vertex_buffers vb{pos, nrm, nullptr, nullptr};
   const bool saved_forced = ctx->forced;
   const wt::LaunchShape saved_shape = ctx->forced_shape;
   float best_t = 1e30f;
   wt::LaunchShape best_s = default_shape(N);
-  const int splits[2] = {1, 2}, ks[3] = {4, 8, 16}, lcs[4] = {64, 32, 16, 8};
+  const int reps = nfl > 1 ? 12 : 5;
+  // frames [0, n) of one timing: frame r on stream r mod nfl, forked from and joined back
+  // to stream 0
+  auto frames = [&](int n) -> wt_status {
+    if (nfl > 1) {
+      if (hipEventRecord(join[0], st[0]) != hipSuccess) return WT_ERR_HIP;
+      for (uint32_t j = 1; j < nfl; ++j)
+        if (hipStreamWaitEvent(st[j], join[0], 0) != hipSuccess) return WT_ERR_HIP;
+    }
+    for (int r = 0; r < n; ++r) {
+      p.waveAnimationTimeInSeconds = 10.0 + r / 60.0;
+      wt_vertex_buffers vb{pos[r % nfl], nrm[r % nfl], nullptr, nullptr};
+      const wt_status s0 = wt_water_displace(ctx, &p, dw, W, with_hm ? &hm : nullptr, &vb, st[r % nfl]);
+      if (s0 != WT_OK) return s0;
+    }
+    for (uint32_t j = 1; j < nfl; ++j) {
+      if (hipEventRecord(join[j], st[j]) != hipSuccess) return WT_ERR_HIP;
+      if (hipStreamWaitEvent(st[0], join[j], 0) != hipSuccess) return WT_ERR_HIP;
+    }
+    return WT_OK;
+  };
+  const int splits[2] = {1, 2}, ks[3] = {4, 8, 16}, lcs[4] = {64, 32, 16, 8}, chunks[3] = {64, 32, 128};
   for (int var : splits)
     for (int k : ks)
-      for (int lc : lcs) {
-        if ((uint32_t)lc > N && lc != 8) continue;
-        ctx->forced = true;
-        ctx->forced_shape = wt::LaunchShape{var, k, lc, 64};
-        p.waveAnimationTimeInSeconds = 10.0;
-        wt_status s0 = wt_water_displace(ctx, &p, dw, W, with_hm ? &hm : nullptr, &vb, st);
-        if (s0 != WT_OK) {
-          ctx->forced = saved_forced;
-          ctx->forced_shape = saved_shape;
-          cleanup();
-          return s0;
+      for (int lc : lcs)
+        for (int ch : chunks) {
+          if ((uint32_t)lc > N && lc != 8) continue;
+          const wt::LaunchShape cand{var, k, lc, ch};
+          if (!valid_shape(cand)) continue;
+          ctx->forced = true;
+          ctx->forced_shape = cand;
+          wt_status s0 = frames((int)nfl);  // first launches of this shape
+          if (s0 == WT_OK && hipStreamSynchronize(st[0]) != hipSuccess) s0 = WT_ERR_HIP;
+          if (s0 != WT_OK) {
+            ctx->forced = saved_forced;
+            ctx->forced_shape = saved_shape;
+            cleanup();
+            return s0 == WT_ERR_HIP ? fail(ctx, s0, "autotune launch") : s0;
+          }
+          WT_TRY(hipEventRecord(e0, st[0]));
+          s0 = frames(reps);
+          WT_TRY(hipEventRecord(e1, st[0]));
+          WT_TRY(hipEventSynchronize(e1));
+          if (s0 != WT_OK) {
+            ctx->forced = saved_forced;
+            ctx->forced_shape = saved_shape;
+            cleanup();
+            return s0 == WT_ERR_HIP ? fail(ctx, s0, "autotune launch") : s0;
+          }
+          float ms = 0.0f;
+          WT_TRY(hipEventElapsedTime(&ms, e0, e1));
+          ms /= reps;
+          if (ms < best_t) {
+            best_t = ms;
+            best_s = cand;
+          }
         }
-        const int reps = 5;
-        WT_TRY(hipEventRecord(e0, st));
-        for (int r = 0; r < reps; ++r) {
-          p.waveAnimationTimeInSeconds = 10.0 + r / 60.0;
-          (void)wt_water_displace(ctx, &p, dw, W, with_hm ? &hm : nullptr, &vb, st);
-        }
-        WT_TRY(hipEventRecord(e1, st));
-        WT_TRY(hipEventSynchronize(e1));
-        float ms = 0.0f;
-        WT_TRY(hipEventElapsedTime(&ms, e0, e1));
-        ms /= reps;
-        if (ms < best_t) {
-          best_t = ms;
-          best_s = ctx->forced_shape;
-        }
-      }
 #undef WT_TRY
   ctx->forced = saved_forced;
   ctx->forced_shape = saved_shape;
@@ -441,11 +484,21 @@ wt_status wt_autotune_forward(wt_ctx* ctx, uint32_t N, uint32_t W, int with_hm, 
     best->wave_split = (uint32_t)best_s.split;
     best->steps_per_lane = (uint32_t)best_s.steps;
     best->lanes_per_row = (uint32_t)best_s.lanes_per_row;
-    best->wave_chunk = 64;
+    best->wave_chunk = (uint32_t)best_s.wave_chunk;
   }
   if (best_ms) *best_ms = best_t;
   cleanup();
   return WT_OK;
+}
+
+wt_status wt_autotune_forward(wt_ctx* ctx, uint32_t N, uint32_t W, int with_hm, wt_launch_config* best,
+                              float* best_ms) {
+  return autotune_forward(ctx, N, W, with_hm, 1, best, best_ms);
+}
+
+wt_status wt_autotune_forward_overlapped(wt_ctx* ctx, uint32_t N, uint32_t W, int with_hm, uint32_t frames_in_flight,
+                                         wt_launch_config* best, float* best_ms) {
+  return autotune_forward(ctx, N, W, with_hm, frames_in_flight, best, best_ms);
 }
 
 wt_status wt_autotune_save(wt_ctx* ctx, const char* path) {

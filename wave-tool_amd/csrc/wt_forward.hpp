@@ -17,7 +17,7 @@
 //    argument in [0, 1] revolution) and steps K vertices with an FP32 rotation (4 VALU
 //    ops per vertex-wave instead of two transcendentals). The rotation per (row, wave)
 //    is computed in FP64 and rounded once.
-//  * Per-wave constants are staged in LDS per chunk of kChunk waves (WaveLds), the
+//  * Per-wave constants are staged in LDS per chunk of CH waves (WaveLds), the
 //    per-row rotations in a per-wavefront LDS table; the next wave's constants are
 //    loaded while the current wave is summed.
 //  * 8 running sums per vertex: x/z displacement, y, and the 5 distinct entries of the
@@ -34,7 +34,7 @@
 
 namespace wt {
 
-constexpr int kChunk = 64;      // waves staged per LDS pass
+constexpr int kChunk = 64;      // default waves staged per LDS pass (CH; 32 and 128 for the S = 2, 64-lane shapes)
 constexpr size_t cmax(size_t a, size_t b) { return a > b ? a : b; }
 constexpr int kWavesPerWG = 4;  // 4 wavefronts of 64 = 256 threads
 
@@ -85,10 +85,11 @@ constexpr double kChebMinRev = 0.015942;
 // Build the step table of the waves this wavefront sums (i = sid, sid + S, ...) for the
 // LR rows of the segment starting at column col_base, in FP64, rounded once. Waves that
 // take the three-term recurrence come first; returns their number (wave-uniform).
-template <int LC, int LR, int S>
+template <int LC, int LR, int S, int CH>
 __device__ __forceinline__ int build_table(const FwdParams& p, int row_base, int col_base, int nw, int sid,
                                            const WaveLds* s_wave, StepEntry* tab, int lane) {
-  constexpr int kMine = kChunk / S;
+  constexpr int kMine = CH / S;
+  static_assert(kMine <= 64, "a wavefront's waves of a chunk fit the 64-lane ballot");
   const int nmine = (nw - sid + S - 1) / S;
   // which waves qualify for the recurrence on every row of the segment
   bool cheb = lane < nmine;
@@ -222,29 +223,29 @@ __device__ __forceinline__ void epilogue_steps(const FwdParams& p, const RowGeom
 
 // LDS of one workgroup of the forward kernels. The wave constants and step tables are
 // dead once the main loop ends, so the S = 2 exchange of partial sums reuses their space.
-template <int K, int LC, int S>
+template <int K, int LC, int S, int CH = kChunk>
 struct SegShared {
   static constexpr int LR = 64 / LC;
   static constexpr int KS = K / S;
   static constexpr int kSegPerWG = kWavesPerWG / S;
-  static constexpr int kTab = LR * (kChunk / S);  // step-table entries per wavefront
-  static constexpr size_t kLoopBytes = sizeof(WaveLds) * kChunk + sizeof(StepEntry) * kWavesPerWG * kTab;
+  static constexpr int kTab = LR * (CH / S);  // step-table entries per wavefront
+  static constexpr size_t kLoopBytes = sizeof(WaveLds) * CH + sizeof(StepEntry) * kWavesPerWG * kTab;
   // partial sums handed to the other wavefront of the pair: [pair][sender][K-KS][8][64]
   static constexpr size_t kXchgFloats = (size_t)(K - KS) * 8 * 64;
   static constexpr size_t kXchgBytes = S > 1 ? sizeof(float) * kSegPerWG * S * kXchgFloats : 0;
   alignas(16) unsigned char raw[kLoopBytes > kXchgBytes ? kLoopBytes : kXchgBytes];
   __device__ WaveLds* wave() { return reinterpret_cast<WaveLds*>(raw); }
   __device__ StepEntry* tab(int wid) {
-    return reinterpret_cast<StepEntry*>(raw + sizeof(WaveLds) * kChunk) + wid * kTab;
+    return reinterpret_cast<StepEntry*>(raw + sizeof(WaveLds) * CH) + wid * kTab;
   }
   __device__ float* xchg(int grp, int sender) { return reinterpret_cast<float*>(raw) + (grp * S + sender) * kXchgFloats; }
 };
-static_assert(sizeof(WaveLds) * kChunk % 16 == 0, "step table alignment");
+static_assert(sizeof(WaveLds) * 32 % 16 == 0, "step table alignment");
 
 // One workgroup's segments of one grid: the whole forward program for the vertices of
 // segment group `seg_group` (kSegPerWG segments). Every thread of the block calls it.
-template <int K, int LC, int S, int kTex>
-__device__ __forceinline__ void forward_segments(const FwdParams& p, long long seg_group, SegShared<K, LC, S>& sh) {
+template <int K, int LC, int S, int kTex, int CH = kChunk>
+__device__ __forceinline__ void forward_segments(const FwdParams& p, long long seg_group, SegShared<K, LC, S, CH>& sh) {
   constexpr int LR = 64 / LC;
   constexpr int KS = K / S;                       // vertices per lane in the epilogue
   constexpr int kSegPerWG = kWavesPerWG / S;
@@ -272,15 +273,15 @@ __device__ __forceinline__ void forward_segments(const FwdParams& p, long long s
 #pragma unroll
     for (int q = 0; q < 8; ++q) acc[j][q] = 0.0f;
 
-  for (int w0 = 0; w0 < p.W; w0 += kChunk) {
-    const int nw = p.W - w0 < kChunk ? p.W - w0 : kChunk;
+  for (int w0 = 0; w0 < p.W; w0 += CH) {
+    const int nw = p.W - w0 < CH ? p.W - w0 : CH;
     __syncthreads();
     stage_chunk(p, w0, nw, s_wave);
     __syncthreads();
-    const int ncheb = build_table<LC, LR, S>(p, row_base, col_base, nw, sid, s_wave, sh.tab(wid), lane);
+    const int ncheb = build_table<LC, LR, S, CH>(p, row_base, col_base, nw, sid, s_wave, sh.tab(wid), lane);
     __syncthreads();
     if (sid >= nw) continue;
-    const StepEntry* tab = sh.tab(wid) + lr * (kChunk / S);
+    const StepEntry* tab = sh.tab(wid) + lr * (CH / S);
     // Software pipeline over this wavefront's table entries (three-term recurrence
     // first, then rotation): the seed of entry k + 1 is formed between the two halves
     // of entry k's steps (scheduling barriers keep the compiler from hoisting its
@@ -346,10 +347,10 @@ __device__ __forceinline__ void forward_segments(const FwdParams& p, long long s
     epilogue_steps<K, LC, KS, (S > 1 ? KS : 0), kTex>(p, g, acc, col0, row, active);
 }
 
-template <int K, int LC, int S, int kTex>
+template <int K, int LC, int S, int kTex, int CH>
 __global__ __launch_bounds__(256) void water_forward_kernel(FwdParams p) {
-  __shared__ SegShared<K, LC, S> sh;
-  forward_segments<K, LC, S, kTex>(p, (long long)blockIdx.x, sh);
+  __shared__ SegShared<K, LC, S, CH> sh;
+  forward_segments<K, LC, S, kTex, CH>(p, (long long)blockIdx.x, sh);
 }
 
 // ------------------------------------------------------------------ batched
@@ -428,13 +429,13 @@ hipError_t launch_batched_mode(const BatchArgs& a, int shape, hipStream_t st) {
 }
 
 // ------------------------------------------------------------------ launch helpers
-template <int K, int LC, int S, int kTex>
+template <int K, int LC, int S, int kTex, int CH = kChunk>
 hipError_t launch_t(const FwdParams& p, hipStream_t st) {
   constexpr int LR = 64 / LC;
   constexpr int kSegPerWG = kWavesPerWG / S;
   const long long segs = (long long)((p.N + LC * K - 1) / (LC * K)) * ((p.N + LR - 1) / LR);
   const long long blocks = (segs + kSegPerWG - 1) / kSegPerWG;
-  hipLaunchKernelGGL((water_forward_kernel<K, LC, S, kTex>), dim3((unsigned)blocks), dim3(256), 0, st, p);
+  hipLaunchKernelGGL((water_forward_kernel<K, LC, S, kTex, CH>), dim3((unsigned)blocks), dim3(256), 0, st, p);
   return hipGetLastError();
 }
 
@@ -444,7 +445,12 @@ template <int kTex>
 hipError_t launch_forward_mode(const FwdParams& p, const LaunchShape& s, hipStream_t st) {
   auto by_split = [&](auto k_tag, auto lc_tag) -> hipError_t {
     constexpr int K = decltype(k_tag)::value, LC = decltype(lc_tag)::value;
-    return s.split == 2 ? launch_t<K, LC, 2, kTex>(p, st) : launch_t<K, LC, 1, kTex>(p, st);
+    if (s.split != 2) return launch_t<K, LC, 1, kTex>(p, st);
+    if constexpr (LC == 64) {  // the wave-chunk variants of the one-row shapes
+      if (s.wave_chunk == 32) return launch_t<K, LC, 2, kTex, 32>(p, st);
+      if (s.wave_chunk == 128) return launch_t<K, LC, 2, kTex, 128>(p, st);
+    }
+    return launch_t<K, LC, 2, kTex>(p, st);
   };
   auto by_lc = [&](auto k_tag) -> hipError_t {
     switch (s.lanes_per_row) {

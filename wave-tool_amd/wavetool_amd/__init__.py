@@ -265,11 +265,18 @@ class Context:
         return dict(wave_split=cfg.wave_split, steps=cfg.steps_per_lane, lanes_per_row=cfg.lanes_per_row,
                     wave_chunk=cfg.wave_chunk)
 
-    def autotune(self, N, W, with_heightmap=True):
+    def autotune(self, N, W, with_heightmap=True, frames_in_flight=1):
+        """Tune the plain kernel's launch shape for (N, W); with frames_in_flight > 1 each
+        shape is timed with that many frames' launches overlapping."""
         cfg = LaunchConfig()
         ms = ctypes.c_float()
-        self._check(lib().wt_autotune_forward(self.handle, N, W, int(bool(with_heightmap)), ctypes.byref(cfg),
-                                              ctypes.byref(ms)))
+        if frames_in_flight == 1:
+            st = lib().wt_autotune_forward(self.handle, N, W, int(bool(with_heightmap)), ctypes.byref(cfg),
+                                           ctypes.byref(ms))
+        else:
+            st = lib().wt_autotune_forward_overlapped(self.handle, N, W, int(bool(with_heightmap)),
+                                                      int(frames_in_flight), ctypes.byref(cfg), ctypes.byref(ms))
+        self._check(st)
         return dict(wave_split=cfg.wave_split, steps=cfg.steps_per_lane, lanes_per_row=cfg.lanes_per_row,
                     wave_chunk=cfg.wave_chunk), ms.value
 

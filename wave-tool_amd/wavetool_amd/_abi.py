@@ -195,6 +195,9 @@ PROTOTYPES = [
     ("wt_autotune_forward", ctypes.c_int,
      [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(LaunchConfig),
       ctypes.POINTER(ctypes.c_float)]),
+    ("wt_autotune_forward_overlapped", ctypes.c_int,
+     [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint32, ctypes.POINTER(LaunchConfig),
+      ctypes.POINTER(ctypes.c_float)]),
     ("wt_autotune_batch", ctypes.c_int,
      [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32),
       ctypes.POINTER(ctypes.c_float)]),
