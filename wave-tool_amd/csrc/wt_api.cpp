@@ -420,7 +420,7 @@ static wt_status autotune_forward(wt_ctx* ctx, uint32_t N, uint32_t W, int with_
   float best_t = 1e30f, default_t = 1e30f;
   wt::LaunchShape best_s = default_shape(N);
   const wt::LaunchShape dflt = default_shape(N);
-  const int reps = nfl > 1 ? 12 : 5;
+  const int reps = nfl > 1 ? 32 : 8;
   // frames [0, n) of one timing: frame r on stream r mod nfl, forked from and joined back
   // to stream 0
   auto frames = [&](int n) -> wt_status {
@@ -482,8 +482,8 @@ static wt_status autotune_forward(wt_ctx* ctx, uint32_t N, uint32_t W, int with_
         }
 #undef WT_TRY
   // shapes within timing noise of each other trade places from run to run: keep the
-  // default shape unless another is more than 2% faster
-  if (default_t <= 1.02f * best_t) {
+  // default shape unless another is more than 3% faster
+  if (default_t <= 1.03f * best_t) {
     best_s = dflt;
     best_t = default_t;
   }
