@@ -43,6 +43,8 @@ struct FitWaveLds {
   float tc;            // 2 cos(omega dt), the three-term recurrence's coefficient
   int orig;            // the wave's index in the problem (its gradient rows)
   float w, wL, om2L;   // w, w / L, omega / (2 L)
+  float qdx, qdz;      // Q Dx, Q Dz
+  float od, op;        // D . (x0, z0) and D-perp . (x0, z0), the grid origin's terms
 };
 
 // The three-term recurrence x[s+1] = 2 cos(d) x[s] - x[s-1] steps (cos, sin) with one FMA
@@ -79,12 +81,13 @@ __device__ __forceinline__ float row_sum(float v) {
   return v;
 }
 
-// gfx950's lane swaps between two registers, summed: with x, y holding row sums (rows
-// r0..r3 of 16 lanes),
+// gfx950's lane swaps between two registers, summed lane by lane. With rows r0..r3 of
+// 16 lanes,
 //   swap32_sum(x, y) = [x.r0 + x.r2, x.r1 + x.r3, y.r0 + y.r2, y.r1 + y.r3],
 //   swap16_sum(x, y) = [x.r0 + x.r1, y.r0 + y.r1, x.r2 + x.r3, y.r2 + y.r3].
-// Two values share each step, so reducing several values across the wavefront takes
-// about half the instructions of one broadcast chain per value.
+// Each step sums two values over half their lanes at once, so four values reduce over
+// the wavefront in two swap32 sums, one swap16 sum and one row sum of the packed result
+// (10 instructions, against 22 with a row sum per value first).
 __device__ __forceinline__ float swap32_sum(float x, float y) {
   const auto p = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(y), false, false);
   return __uint_as_float(p[0]) + __uint_as_float(p[1]);
@@ -133,6 +136,10 @@ __device__ __forceinline__ void stage_fit_wave(const FitDesc& P, int i, FitWaveL
   o->w = (float)w;
   o->wL = (float)(w / L);
   o->om2L = (float)(om / (2.0 * L));
+  o->qdx = (float)(Q * ca);
+  o->qdz = (float)(Q * sa);
+  o->od = (float)(ca * P.x0 + sa * P.z0);
+  o->op = (float)(ca * P.z0 - sa * P.x0);
 }
 
 // (c, sn) <- the next snapshot's (cos, sin). Rotation form: by omega dt. Three-term form:
@@ -188,17 +195,18 @@ __device__ __forceinline__ void forward_wave(const FitWaveLds& wv, const float (
 }
 
 // Backward of one wave: six sums over snapshots and three sums of their running values
-// per point, then the five derivatives, added over the thread's points, reduced over
-// the wavefront (row sums, then lane swaps that leave dA, dalpha, dL, dQ in rows 0..3
-// and dphi0 in row 0) and written to the wave's rows of s_red.
+// per point, then the five derivatives, added over the thread's points and reduced over
+// the wavefront: lane swaps first (each step sums two registers), then one row sum of
+// the packed register, which leaves dA, dalpha, dL, dQ in rows 0..3 and dphi0 in every
+// row; written to the wave's rows of s_red.
 template <int SMAX, int KP, bool kCheb>
 __device__ __forceinline__ void backward_wave(const FitDesc& P, const FitWaveLds& wv, const float (&c0)[KP],
                                               const float (&s0)[KP], const double (&x)[KP], const double (&z)[KP],
                                               const float (&r)[KP][SMAX][3], float* s_red, int lane, int wid) {
-  const float x0f = (float)P.x0, z0f = (float)P.z0;
-  const float dtf = (float)P.dt, Sf = (float)SMAX, t0f = (float)P.t0;
+  const float dtf = (float)P.dt;
+  // t0 + SMAX dt: the time weight of the snapshot sums (below)
+  const float tS = (float)(P.t0 + P.dt * SMAX);
   const float A = wv.A, Q = wv.Q, Dx = wv.Dx, Dz = wv.Dz;
-  const float origin_dot = fmaf(Dx, x0f, Dz * z0f), origin_perp = fmaf(-Dz, x0f, Dx * z0f);
   float gA = 0.0f, gL = 0.0f, gAl = 0.0f, gQ = 0.0f, gP = 0.0f;
 #pragma unroll
   for (int k = 0; k < KP; ++k) {
@@ -218,30 +226,30 @@ __device__ __forceinline__ void backward_wave(const FitDesc& P, const FitWaveLds
       vzs += szs;
       step_snapshot<kCheb>(s, c, sn, cp, sp, wv);
     }
-    // time-weighted sums: sum_s (s dt) f_s = dt (SMAX U - V), the residuals of the
-    // snapshots past S being zero
-    const float tsyc = dtf * fmaf(Sf, syc, -vyc);
-    const float tsxs = dtf * fmaf(Sf, sxs, -vxs);
-    const float tszs = dtf * fmaf(Sf, szs, -vzs);
-    const float sec = Dx * sxc + Dz * szc;    // sum of (r . D) cos
-    const float sepc = -Dz * sxc + Dx * szc;  // sum of (r . D-perp) cos
-    const float g0 = A * (syc - Q * (Dx * sxs + Dz * szs));     // sum of r . dd/dtheta
-    const float gt = A * (tsyc - Q * (Dx * tsxs + Dz * tszs));  // sum of (t - t0) r . dd/dtheta
-    const float g0t = fmaf(t0f, g0, gt);                        // sum of t r . dd/dtheta
-    // D.x and D-perp.x split into the problem origin term and the local term
+    // r . dd/dtheta = A (ry cos - Q (D . r) sin): m0 is its sum over snapshots / A, m1
+    // the same of the running sums. With t_s = t0 + s dt and sum_s s f_s = SMAX U - V
+    // (U a sum, V the sum of its running values; the residuals past S are zero),
+    // sum_s t_s f_s = (t0 + SMAX dt) U - dt V.
+    const float m0 = syc - fmaf(wv.qdx, sxs, wv.qdz * szs);
+    const float m1 = vyc - fmaf(wv.qdx, vxs, wv.qdz * vzs);
+    const float g0 = A * m0;                        // sum of r . dd/dtheta
+    const float g0t = A * fmaf(tS, m0, -dtf * m1);  // sum of t r . dd/dtheta
+    const float sec = fmaf(Dx, sxc, Dz * szc);      // sum of (r . D) cos
+    const float sepc = fmaf(Dx, szc, -Dz * sxc);    // sum of (r . D-perp) cos
+    // D . x and D-perp . x: the local offset from the grid origin plus the origin's term
     const float xf = (float)(x[k] - P.x0), zf = (float)(z[k] - P.z0);
-    const float dotx = fmaf(Dx, xf, Dz * zf) + origin_dot;
-    const float perpx = fmaf(-Dz, xf, Dx * zf) + origin_perp;
+    const float dotx = fmaf(Dx, xf, fmaf(Dz, zf, wv.od));
+    const float perpx = fmaf(Dx, zf, fmaf(-Dz, xf, wv.op));
     // inactive points have zero residuals, so every sum above is zero for them
     gA += fmaf(Q, sec, sys);
-    gL += -wv.wL * dotx * g0 - wv.om2L * g0t;
+    gL -= fmaf(wv.wL * dotx, g0, wv.om2L * g0t);
     gAl += fmaf(wv.w * perpx, g0, Q * A * sepc);
     gQ += A * sec;
     gP += g0;
   }
-  const float t = swap16_sum(swap32_sum(row_sum(gA), row_sum(gL)), swap32_sum(row_sum(gAl), row_sum(gQ)));
-  const float p = row_sum(gP);
-  const float tp = swap16_sum(swap32_sum(p, p), swap32_sum(p, p));
+  const float t = row_sum(swap16_sum(swap32_sum(gA, gL), swap32_sum(gAl, gQ)));
+  const float pp = swap32_sum(gP, gP);
+  const float tp = row_sum(swap16_sum(pp, pp));
   if ((lane & 15) == 15) {
     const int row = lane >> 4;
     float* o = s_red + (wv.orig * 5) * 4 + wid;
@@ -370,7 +378,8 @@ __device__ __forceinline__ int stage_fit_waves(const FitDesc& P, FitWaveLds* s_w
 // One launch covers the blocks [base, base + gridDim.x) of the batch's block -> problem
 // map, all of one snapshot class.
 template <int SMAX, int KP>
-__global__ __launch_bounds__(kFitThreads) void adjoint_kernel(const FitDesc* __restrict__ probs,
+__global__ __launch_bounds__(kFitThreads) __attribute__((amdgpu_waves_per_eu(kFitWavesPerSimd[(SMAX - 1) / 4])))
+void adjoint_kernel(const FitDesc* __restrict__ probs,
                                                               const int* __restrict__ block_prob, long long base,
                                                               int bump_step) {
   __shared__ FitWaveLds s_wave[kFitMaxWaves];

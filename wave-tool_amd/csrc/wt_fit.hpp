@@ -20,6 +20,11 @@ constexpr int kFitThreads = 256;
 constexpr int kFitClasses = 8;
 constexpr int kFitClassMaxS[kFitClasses] = {4, 8, 12, 16, 20, 24, 28, 32};
 constexpr int kFitPointsPerThread[kFitClasses] = {2, 2, 1, 1, 1, 1, 1, 1};
+// Wavefronts per SIMD each class's adjoint kernel is compiled for (1: no request). The
+// classes listed fit one more wavefront per SIMD than the register allocator's own
+// choice gives, with no spills and the same instructions (4: 84 -> 80 VGPRs, 12: 83 ->
+// 80, 20: 101 -> 96); for the others the next step spills.
+constexpr int kFitWavesPerSimd[kFitClasses] = {6, 1, 6, 1, 5, 1, 1, 1};
 inline int fit_class(int S) { return (S - 1) / 4; }
 
 // One problem of a batch as the kernels see it. Device pointers are owned by the
