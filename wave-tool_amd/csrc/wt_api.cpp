@@ -241,6 +241,8 @@ wt_status wt_water_displace(wt_ctx* ctx, const wt_water_params* p, const wt_gers
   e.s_h = p->heightmapSampleScale;
   e.s_d = p->heightmapDisplacementScale;
   e.bounce = p->verticalBounceWaveDisplacement;
+  e.two_sd = 2.0f * e.s_d;
+  e.bump_k = 2.0f * e.s_d * e.s_h;
   e.pos = out->position;
   e.nrm = out->normal;
   e.tan = out->tangent;

@@ -170,6 +170,7 @@ __device__ __forceinline__ float heightmap_eval(HmTap h, int W, int H, float* dd
 struct EpilogueArgs {
   float cam[3];
   float s_h, s_d, bounce;
+  float two_sd, bump_k;       // 2 s_d and 2 s_d s_h, formed once per launch (2.0f * s_d * s_h)
   double hm_off_s, hm_off_t;  // texture-space scroll offsets, already reduced to one period
   const void* texels;
   int tex_w, tex_h, tex_fmt, tex_wrap;
@@ -238,8 +239,8 @@ __device__ __forceinline__ void vertex_finish(const EpilogueArgs& e, const Verte
   if constexpr (M != kTexNone) {
     float dds, ddt;
     const float r = heightmap_eval<M>(v.tap, e.tex_w, e.tex_h, &dds, &ddt);
-    py += e.s_d * (2.0f * (r - 0.5f));  // water-grid.vert:108-110
-    const float k = 2.0f * e.s_d * e.s_h;
+    py += e.two_sd * (r - 0.5f);  // water-grid.vert:108-110: s_d * 2 (r - 0.5)
+    const float k = e.bump_k;
     const float hpx = k * dds, hpz = -k * ddt;  // d(bump)/dPx, d(bump)/dPz
     ty += hpx * tx + hpz * tz;
     by += hpx * bx + hpz * bz;
@@ -255,8 +256,9 @@ __device__ __forceinline__ void vertex_finish(const EpilogueArgs& e, const Verte
   // the outputs are written once per frame and not read back by this kernel: stream
   // them past the caches (4% faster on the 1024^2 x 64-wave frame)
   // byte offset within the row as a 32-bit value (a row is at most 32768 x 12 bytes), so
-  // each store adds it to the row pointer without 64-bit shifts
-  const uint32_t b = 12u * col;
+  // each store adds it to the row pointer without 64-bit shifts; col < 2^24, so the
+  // full-rate 24-bit multiply (a 32-bit one issues at quarter rate)
+  const uint32_t b = __umul24(12u, col);
   store3_nt(row_at(ro.pos, b), px, py, pz);
   store3_nt(row_at(ro.nrm, b), nx, ny, nz);
   if (ro.tan) {

@@ -379,6 +379,8 @@ __device__ __forceinline__ FwdParams patch_params(const BatchArgs& a, const Patc
   e.s_h = pd.s_h;
   e.s_d = pd.s_d;
   e.bounce = pd.bounce;
+  e.two_sd = 2.0f * pd.s_d;
+  e.bump_k = 2.0f * pd.s_d * pd.s_h;
   e.texels = a.texels;
   e.tex_w = a.tex_w; e.tex_h = a.tex_h; e.tex_fmt = a.tex_fmt; e.tex_wrap = a.tex_wrap;
   // texture-space scroll offset -s_h v t reduced to one wrap period in FP64
