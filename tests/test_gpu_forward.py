@@ -84,7 +84,9 @@ def test_grid256_w4_parity(torch_cuda, wtctx):
     assert res["tan_max"] <= NRM_TOL and res["bit_max"] <= NRM_TOL
 
 
-BENCH_SHAPE = (2, 16, 64)  # the shape bench.py reports for this workload (wave_split, steps, lanes per row)
+# besides the default (K8), the shape the benchmark's autotuner has picked for this workload on some
+# devices (wave_split, steps, lanes per row); the "autotuned" case runs whatever it picks on this one
+BENCH_SHAPE = (2, 16, 64)
 
 
 @pytest.mark.parametrize("frame", [0, 239])
