@@ -106,40 +106,46 @@ __device__ __forceinline__ double fit_wave_step(const FitDesc& P, int i) {
   return (step - floor(step)) * kTwoPi;
 }
 
-__device__ __forceinline__ int fit_wave_cheb(const FitDesc& P, int i) {
-  return fabs(sin(fit_wave_step(P, i))) >= kFitChebMinSin;
-}
-
-__device__ __forceinline__ void stage_fit_wave(const FitDesc& P, int i, FitWaveLds* o) {
+// One wave's constants as the kernels stage them, and whether it takes the three-term
+// recurrence (|sin omega dt| >= kFitChebMinSin).
+__device__ __forceinline__ FitWaveLds fit_wave_constants(const FitDesc& P, int i, bool* cheb) {
   const float* pr = P.params + 5 * i;
   const double A = pr[0], L = pr[1] > 1e-3f ? pr[1] : 1e-3f, al = pr[2], Q = pr[3], ph0 = pr[4];
   const double w = kTwoPi / L, om = sqrt(P.g * w);
   double sa, ca;
   sincos(al, &sa, &ca);
-  o->kx = w * ca * kInvTwoPi;
-  o->kz = w * sa * kInvTwoPi;
+  FitWaveLds o;
+  o.kx = w * ca * kInvTwoPi;
+  o.kz = w * sa * kInvTwoPi;
   const double ph = (om * P.t0 + ph0) * kInvTwoPi;
-  o->c0 = ph - floor(ph);
-  o->om_rev = om * kInvTwoPi;
+  o.c0 = ph - floor(ph);
+  o.om_rev = om * kInvTwoPi;
   double sn, cs;
   sincos(fit_wave_step(P, i), &sn, &cs);
-  o->rc = (float)cs;
-  o->rs = (float)sn;
-  o->tc = (float)(2.0 * cs);
-  o->orig = i;
-  o->A = (float)A;
-  o->Q = (float)Q;
-  o->Dx = (float)ca;
-  o->Dz = (float)sa;
-  o->qadx = (float)(Q * A * ca);
-  o->qadz = (float)(Q * A * sa);
-  o->w = (float)w;
-  o->wL = (float)(w / L);
-  o->om2L = (float)(om / (2.0 * L));
-  o->qdx = (float)(Q * ca);
-  o->qdz = (float)(Q * sa);
-  o->od = (float)(ca * P.x0 + sa * P.z0);
-  o->op = (float)(ca * P.z0 - sa * P.x0);
+  *cheb = fabs(sn) >= kFitChebMinSin;
+  o.rc = (float)cs;
+  o.rs = (float)sn;
+  o.tc = (float)(2.0 * cs);
+  o.orig = i;
+  o.A = (float)A;
+  o.Q = (float)Q;
+  o.Dx = (float)ca;
+  o.Dz = (float)sa;
+  o.qadx = (float)(Q * A * ca);
+  o.qadz = (float)(Q * A * sa);
+  o.w = (float)w;
+  o.wL = (float)(w / L);
+  o.om2L = (float)(om / (2.0 * L));
+  o.qdx = (float)(Q * ca);
+  o.qdz = (float)(Q * sa);
+  o.od = (float)(ca * P.x0 + sa * P.z0);
+  o.op = (float)(ca * P.z0 - sa * P.x0);
+  return o;
+}
+
+__device__ __forceinline__ void stage_fit_wave(const FitDesc& P, int i, FitWaveLds* o) {
+  bool cheb;
+  *o = fit_wave_constants(P, i, &cheb);
 }
 
 // (c, sn) <- the next snapshot's (cos, sin). Rotation form: by omega dt. Three-term form:
@@ -330,9 +336,12 @@ __device__ __forceinline__ void adjoint_body(const FitDesc& P, int local_block, 
   forward_waves<SMAX, KP, true>(s_wave, 0, ncheb, x, z, r);
   forward_waves<SMAX, KP, false>(s_wave, ncheb, P.W, x, z, r);
   // residuals and loss
-  float loss = 0.0f;
+  float sq = 0.0f;  // sum of squared residuals; the loss is half of it
 #pragma unroll
   for (int k = 0; k < KP; ++k) {
+    // the point's target record in snapshot 0; snapshot s is s V records further (a
+    // wave-uniform offset)
+    const float* tgp = P.target + (size_t)vv[k] * 3;
 #pragma unroll
     for (int s = 0; s < SMAX; ++s) {
       // snapshots past S (and points past the grid) get zero residuals; their loads
@@ -340,17 +349,17 @@ __device__ __forceinline__ void adjoint_body(const FitDesc& P, int local_block, 
       // factor rather than a select: a select on a loaded value is turned into a branch
       // around the load, which then waits for memory once per snapshot.
       const float use = active[k] && s < S ? 1.0f : 0.0f;
-      const float* tg = P.target + ((size_t)(s < S ? s : S - 1) * V + vv[k]) * 3;
+      const float* tg = tgp + (size_t)(s < S ? s : S - 1) * (size_t)(3 * V);
       const float t0 = tg[0], t1 = tg[1], t2 = tg[2];
       r[k][s][0] = (r[k][s][0] - t0) * use;
       r[k][s][1] = (r[k][s][1] - t1) * use;
       r[k][s][2] = (r[k][s][2] - t2) * use;
-      loss = fmaf(0.5f * r[k][s][0], r[k][s][0], loss);
-      loss = fmaf(0.5f * r[k][s][1], r[k][s][1], loss);
-      loss = fmaf(0.5f * r[k][s][2], r[k][s][2], loss);
+      sq = fmaf(r[k][s][0], r[k][s][0], sq);
+      sq = fmaf(r[k][s][1], r[k][s][1], sq);
+      sq = fmaf(r[k][s][2], r[k][s][2], sq);
     }
   }
-  const float lsum = wave_sum63(loss);
+  const float lsum = 0.5f * wave_sum63(sq);
   if (lane == 63) s_red[(P.W * 5) * 4 + wid] = lsum;
 
   // backward
@@ -359,20 +368,33 @@ __device__ __forceinline__ void adjoint_body(const FitDesc& P, int local_block, 
 }
 
 // Stage the problem's waves into LDS, three-term-recurrence waves first (each group in
-// wave order); returns their number. Every thread of the block calls it.
-__device__ __forceinline__ int stage_fit_waves(const FitDesc& P, FitWaveLds* s_wave, int* s_flag) {
-  for (int i = threadIdx.x; i < P.W; i += blockDim.x) s_flag[i] = fit_wave_cheb(P, i);
-  __syncthreads();
-  int ncheb = 0;
-  for (int j = 0; j < P.W; ++j) ncheb += s_flag[j];
-  for (int i = threadIdx.x; i < P.W; i += blockDim.x) {
-    int before = 0;
-    for (int j = 0; j < i; ++j) before += s_flag[j];
-    const int pos = s_flag[i] ? before : ncheb + (i - before);
-    stage_fit_wave(P, i, &s_wave[pos]);
+// wave order); returns their number. Wavefront 0 does it, lane l taking waves l and
+// l + 64 (kFitMaxWaves = 128): the recurrence flags of both halves are balloted, so each
+// wave's position comes from two population counts, with no pass over the flags.
+__device__ __forceinline__ int stage_fit_waves(const FitDesc& P, FitWaveLds* s_wave, int* s_ncheb) {
+  static_assert(kFitMaxWaves <= 128, "two waves per lane of one wavefront");
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    const int i0 = lane, i1 = lane + 64;
+    bool c0 = false, c1 = false;
+    FitWaveLds w0, w1;
+    if (i0 < P.W) w0 = fit_wave_constants(P, i0, &c0);
+    if (i1 < P.W) w1 = fit_wave_constants(P, i1, &c1);
+    const uint64_t b0 = __ballot(c0), b1 = __ballot(c1);
+    const uint64_t below = (1ull << lane) - 1ull;  // lanes under this one
+    const int n0 = __popcll(b0), ncheb = n0 + __popcll(b1);
+    if (i0 < P.W) {
+      const int before = __popcll(b0 & below);  // recurrence waves ahead of i0
+      s_wave[c0 ? before : ncheb + (i0 - before)] = w0;
+    }
+    if (i1 < P.W) {
+      const int before = n0 + __popcll(b1 & below);
+      s_wave[c1 ? before : ncheb + (i1 - before)] = w1;
+    }
+    if (lane == 0) *s_ncheb = ncheb;
   }
   __syncthreads();
-  return ncheb;
+  return *s_ncheb;
 }
 
 // One launch covers the blocks [base, base + gridDim.x) of the batch's block -> problem
@@ -384,14 +406,14 @@ void adjoint_kernel(const FitDesc* __restrict__ probs,
                                                               int bump_step) {
   __shared__ FitWaveLds s_wave[kFitMaxWaves];
   __shared__ float s_red[(kFitMaxWaves * 5 + 1) * 4];
-  __shared__ int s_flag[kFitMaxWaves];
+  __shared__ int s_ncheb;
   const long long gb = base + blockIdx.x;
   // the problem and block tables do not change during the launch: scalar loads, so the
   // descriptor sits in SGPRs rather than VGPRs
   const FitDesc P = load_uniform(probs, load_uniform(block_prob, gb));
   const int local_block = (int)(gb - P.block_offset);
   if (bump_step && local_block == 0 && threadIdx.x == 0) P.step[0] += 1;  // Adam step of this iteration
-  const int ncheb = stage_fit_waves(P, s_wave, s_flag);
+  const int ncheb = stage_fit_waves(P, s_wave, &s_ncheb);
   adjoint_body<SMAX, KP>(P, local_block, s_wave, ncheb, s_red);
   __syncthreads();
   // fixed-order sum over the 4 wavefronts, one partial per (row, block)
