@@ -571,8 +571,10 @@ def run_extra(name, torch, wt, scenes, args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    # 20 warm-up and 50 timed steps (about 0.35 s of frames): with 2 and 5 the same frames
+    # read up to 4% slower on some devices, before the clocks had settled
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--launch-shape", default="",
