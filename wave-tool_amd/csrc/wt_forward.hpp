@@ -17,9 +17,10 @@
 //    argument in [0, 1] revolution) and steps K vertices with an FP32 rotation (4 VALU
 //    ops per vertex-wave instead of two transcendentals). The rotation per (row, wave)
 //    is computed in FP64 and rounded once.
-//  * Per-wave constants are staged in LDS per chunk of CH waves (WaveLds), the
-//    per-row rotations in a per-wavefront LDS table; the next wave's constants are
-//    loaded while the current wave is summed.
+//  * Waves are taken CH at a time. For each chunk every wavefront builds an LDS table
+//    of its (row, wave) entries straight from the wave block (L2-resident): the
+//    rotation, the seed and the wave's coefficients; the next entry is read while the
+//    current wave is summed.
 //  * 8 running sums per vertex: x/z displacement, y, and the 5 distinct entries of the
 //    analytic Jacobian, all VALU FMAs with VGPR operands (no SGPR operand, which would
 //    block dual issue).
@@ -52,11 +53,6 @@ __device__ __forceinline__ RowGeom row_geom(const FwdParams& p, int row) {
   return g;
 }
 
-// Stage waves [w0, w0 + nw) of the frame into LDS (every thread of the block calls).
-__device__ __forceinline__ void stage_chunk(const FwdParams& p, int w0, int nw, WaveLds* s_wave) {
-  for (int t = threadIdx.x; t < nw; t += blockDim.x) stage_wave(p.waves[w0 + t], p.time, &s_wave[t]);
-}
-
 // Per (row of a segment, wave of a wavefront) entry of the step table.
 //  rot:    cos, sin, -sin of the phase advance over LC columns, and 2 cos of it;
 //  seed:   phase of the segment's first column (revolutions, [0, 1)), the per-column
@@ -87,14 +83,15 @@ constexpr double kChebMinRev = 0.015942;
 // take the three-term recurrence come first; returns their number (wave-uniform).
 template <int LC, int LR, int S, int CH>
 __device__ __forceinline__ int build_table(const FwdParams& p, int row_base, int col_base, int nw, int sid,
-                                           const WaveLds* s_wave, StepEntry* tab, int lane) {
+                                           int w0, StepEntry* tab, int lane) {
   constexpr int kMine = CH / S;
   static_assert(kMine <= 64, "a wavefront's waves of a chunk fit the 64-lane ballot");
   const int nmine = (nw - sid + S - 1) / S;
   // which waves qualify for the recurrence on every row of the segment
   bool cheb = lane < nmine;
   if (cheb) {
-    const WaveLds& w = s_wave[sid + lane * S];
+    WaveLds w;
+    stage_wave(p.waves[w0 + sid + lane * S], p.time, &w);
 #pragma unroll
     for (int r = 0; r < LR; ++r) {
       int row = row_base + r;
@@ -116,7 +113,8 @@ __device__ __forceinline__ int build_table(const FwdParams& p, int row_base, int
     int row = row_base + r;
     row = row < p.N ? row : p.N - 1;
     const RowGeom g = row_geom(p, row);
-    const WaveLds& w = s_wave[i];
+    WaveLds w;
+    stage_wave(p.waves[w0 + i], p.time, &w);
     const double q = fma(w.kx, g.xs, w.kz * g.zs);  // revolutions per column
     const double qf = q - floor(q);
     const double step = qf * (double)LC;
@@ -229,18 +227,16 @@ struct SegShared {
   static constexpr int KS = K / S;
   static constexpr int kSegPerWG = kWavesPerWG / S;
   static constexpr int kTab = LR * (CH / S);  // step-table entries per wavefront
-  static constexpr size_t kLoopBytes = sizeof(WaveLds) * CH + sizeof(StepEntry) * kWavesPerWG * kTab;
+  static constexpr size_t kLoopBytes = sizeof(StepEntry) * kWavesPerWG * kTab;
   // partial sums handed to the other wavefront of the pair: [pair][sender][K-KS][8][64]
   static constexpr size_t kXchgFloats = (size_t)(K - KS) * 8 * 64;
   static constexpr size_t kXchgBytes = S > 1 ? sizeof(float) * kSegPerWG * S * kXchgFloats : 0;
   alignas(16) unsigned char raw[kLoopBytes > kXchgBytes ? kLoopBytes : kXchgBytes];
-  __device__ WaveLds* wave() { return reinterpret_cast<WaveLds*>(raw); }
   __device__ StepEntry* tab(int wid) {
-    return reinterpret_cast<StepEntry*>(raw + sizeof(WaveLds) * CH) + wid * kTab;
+    return reinterpret_cast<StepEntry*>(raw) + wid * kTab;
   }
   __device__ float* xchg(int grp, int sender) { return reinterpret_cast<float*>(raw) + (grp * S + sender) * kXchgFloats; }
 };
-static_assert(sizeof(WaveLds) * 32 % 16 == 0, "step table alignment");
 
 // One workgroup's segments of one grid: the whole forward program for the vertices of
 // segment group `seg_group` (kSegPerWG segments). Every thread of the block calls it.
@@ -249,7 +245,6 @@ __device__ __forceinline__ void forward_segments(const FwdParams& p, long long s
   constexpr int LR = 64 / LC;
   constexpr int KS = K / S;                       // vertices per lane in the epilogue
   constexpr int kSegPerWG = kWavesPerWG / S;
-  WaveLds* s_wave = sh.wave();
 
   const int lane = threadIdx.x & 63;
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
@@ -276,9 +271,7 @@ __device__ __forceinline__ void forward_segments(const FwdParams& p, long long s
   for (int w0 = 0; w0 < p.W; w0 += CH) {
     const int nw = p.W - w0 < CH ? p.W - w0 : CH;
     __syncthreads();
-    stage_chunk(p, w0, nw, s_wave);
-    __syncthreads();
-    const int ncheb = build_table<LC, LR, S, CH>(p, row_base, col_base, nw, sid, s_wave, sh.tab(wid), lane);
+    const int ncheb = build_table<LC, LR, S, CH>(p, row_base, col_base, nw, sid, w0, sh.tab(wid), lane);
     __syncthreads();
     if (sid >= nw) continue;
     const StepEntry* tab = sh.tab(wid) + lr * (CH / S);
