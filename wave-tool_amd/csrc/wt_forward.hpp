@@ -87,6 +87,42 @@ __device__ __forceinline__ int build_table(const FwdParams& p, int row_base, int
   constexpr int kMine = CH / S;
   static_assert(kMine <= 64, "a wavefront's waves of a chunk fit the 64-lane ballot");
   const int nmine = (nw - sid + S - 1) / S;
+  const uint64_t valid = nmine >= 64 ? ~0ull : ((1ull << nmine) - 1ull);
+  if constexpr (LR == 1) {
+    // one row per segment: lane m forms the entry of its wave once and takes the
+    // recurrence flag from the same step (frac(LC q) = frac(LC frac(q)) exactly, LC
+    // being a power of two), then the ballot places the entries
+    const bool mine = lane < nmine;
+    bool cheb = false;
+    StepEntry t;
+    if (mine) {
+      const RowGeom g = row_geom(p, row_base < p.N ? row_base : p.N - 1);
+      WaveLds w;
+      stage_wave(p.waves[w0 + sid + lane * S], p.time, &w);
+      const double q = fma(w.kx, g.xs, w.kz * g.zs);  // revolutions per column
+      const double qf = q - floor(q);
+      const double step = qf * (double)LC;
+      const double sf = step - floor(step);
+      const double h = 2.0 * sf;
+      const double f = 0.5 * (h - floor(h));  // step mod 1/2, in revolutions
+      cheb = f >= kChebMinRev && f <= 0.5 - kChebMinRev;
+      float cr, sr;
+      sincos_rev(sf, &cr, &sr);
+      const double ph = fma((double)col_base, q, fma(w.kx, g.xa, fma(w.kz, g.za, w.c0)));
+      const double qh = floor(qf * 262144.0) * (1.0 / 262144.0);
+      t.rot = make_float4(cr, sr, -sr, 2.0f * cr);
+      t.seed = make_float4((float)(ph - floor(ph)), (float)qh, (float)(qf - qh), 0.0f);
+      t.cc = make_float4(w.cc[0], w.cc[1], w.cc[2], w.cc[3]);
+      t.cs = make_float4(w.cs[0], w.cs[1], w.cs[2], w.cs[3]);
+    }
+    const uint64_t cmask = __ballot(cheb) & valid;
+    const int ncheb = __popcll(cmask);
+    if (mine) {
+      const uint64_t mb = (1ull << lane) - 1ull;
+      tab[cheb ? __popcll(cmask & mb) : ncheb + __popcll(~cmask & valid & mb)] = t;
+    }
+    return ncheb;
+  }
   // which waves qualify for the recurrence on every row of the segment
   bool cheb = lane < nmine;
   if (cheb) {
@@ -103,7 +139,6 @@ __device__ __forceinline__ int build_table(const FwdParams& p, int row_base, int
       cheb = cheb && f >= kChebMinRev && f <= 0.5 - kChebMinRev;
     }
   }
-  const uint64_t valid = nmine >= 64 ? ~0ull : ((1ull << nmine) - 1ull);
   const uint64_t cmask = __ballot(cheb) & valid;
   const int ncheb = __popcll(cmask);
   for (int e = lane; e < LR * nmine; e += 64) {
