@@ -236,22 +236,41 @@ __device__ __forceinline__ void steps_chebyshev(float (&acc)[K][8], float& c, fl
 
 // Epilogue of vertices [JB, JB + KS) of a lane: all texel loads first, then the per-vertex
 // math and stores (JB is a compile-time constant so the sums stay in registers).
-template <int K, int LC, int KS, int JB, int kTex>
-__device__ __forceinline__ void epilogue_steps(const FwdParams& p, const RowGeom& g, const float (&acc)[K][8],
+template <int KS, int LC, int kTex>
+__device__ __forceinline__ void epilogue_steps(const FwdParams& p, const RowGeom& g, const float (&fin)[KS][8],
                                                int col0, int row, bool active) {
   VertexPending pend[KS];
 #pragma unroll
   for (int u = 0; u < KS; ++u) {
-    const int col = col0 + (JB + u) * LC;
+    const int col = col0 + u * LC;
     const double x = fma((double)col, g.xs, g.xa), z = fma((double)col, g.zs, g.za);
-    pend[u] = vertex_fetch<kTex>(p.ep, x, z, acc[JB + u]);
+    pend[u] = vertex_fetch<kTex>(p.ep, x, z, fin[u]);
   }
   const RowOut ro = row_out(p.ep, (size_t)(row < p.N ? row : 0) * (size_t)p.N);
 #pragma unroll
   for (int u = 0; u < KS; ++u) {
-    const int col = col0 + (JB + u) * LC;
-    if (active && row < p.N && col < p.N) vertex_finish<kTex>(p.ep, pend[u], acc[JB + u], ro, (uint32_t)col);
+    const int col = col0 + u * LC;
+    if (active && row < p.N && col < p.N) vertex_finish<kTex>(p.ep, pend[u], fin[u], ro, (uint32_t)col);
   }
+}
+
+// The S = 2 exchange of partial sums ([step][sum][lane] layout in LDS): steps
+// [J0, J0 + KS) of acc out, and the sums of the KS steps from J1 finished with the
+// partner's (fin = acc + theirs). J0 and J1 are template constants, so each wavefront's
+// code path addresses its registers directly (no per-value select on the wavefront).
+template <int K, int KS, int J0>
+__device__ __forceinline__ void xchg_put(const float (&acc)[K][8], float* out, int lane) {
+#pragma unroll
+  for (int j = 0; j < KS; ++j)
+#pragma unroll
+    for (int q = 0; q < 8; ++q) out[(j * 8 + q) * 64 + lane] = acc[J0 + j][q];
+}
+template <int K, int KS, int J1>
+__device__ __forceinline__ void xchg_finish(const float (&acc)[K][8], const float* in, int lane, float (&fin)[KS][8]) {
+#pragma unroll
+  for (int j = 0; j < KS; ++j)
+#pragma unroll
+    for (int q = 0; q < 8; ++q) fin[j][q] = acc[J1 + j][q] + in[(j * 8 + q) * 64 + lane];
 }
 
 // LDS of one workgroup of the forward kernels. The wave constants and step tables are
@@ -347,32 +366,30 @@ __device__ __forceinline__ void forward_segments(const FwdParams& p, long long s
   }
   const RowGeom g = row_geom(p, row < p.N ? row : p.N - 1);
 
-  if (S > 1) {
+  // the KS vertices this wavefront finishes: steps [sid * KS, sid * KS + KS)
+  float fin[KS][8];
+  if constexpr (S > 1) {
     // S = 2: wavefront 0 keeps vertices [0, K/2), wavefront 1 keeps [K/2, K); each hands
     // the other half of its partial sums over through LDS and adds the half it receives
     __syncthreads();
     float* mine = sh.xchg(grp, sid);
-#pragma unroll
-    for (int j = 0; j < KS; ++j) {
-      const int jo = sid == 0 ? j + KS : j;  // the step owned by the other wavefront
-#pragma unroll
-      for (int q = 0; q < 8; ++q) mine[(j * 8 + q) * 64 + lane] = acc[jo][q];
-    }
+    if (sid == 0)
+      xchg_put<K, KS, KS>(acc, mine, lane);
+    else
+      xchg_put<K, KS, 0>(acc, mine, lane);
     __syncthreads();
     const float* theirs = sh.xchg(grp, sid ^ 1);
+    if (sid == 0)
+      xchg_finish<K, KS, 0>(acc, theirs, lane, fin);
+    else
+      xchg_finish<K, KS, KS>(acc, theirs, lane, fin);
+  } else {
 #pragma unroll
-    for (int j = 0; j < KS; ++j) {
-      const int jm = sid == 0 ? j : j + KS;  // my step
+    for (int j = 0; j < KS; ++j)
 #pragma unroll
-      for (int q = 0; q < 8; ++q) acc[jm][q] += theirs[(j * 8 + q) * 64 + lane];
-    }
+      for (int q = 0; q < 8; ++q) fin[j][q] = acc[j][q];
   }
-
-  // epilogue for the KS vertices this wavefront owns (steps [sid*KS, sid*KS + KS))
-  if (S == 1 || sid == 0)
-    epilogue_steps<K, LC, KS, 0, kTex>(p, g, acc, col0, row, active);
-  else
-    epilogue_steps<K, LC, KS, (S > 1 ? KS : 0), kTex>(p, g, acc, col0, row, active);
+  epilogue_steps<KS, LC, kTex>(p, g, fin, col0 + sid * KS * LC, row, active);
 }
 
 template <int K, int LC, int S, int kTex, int CH>
