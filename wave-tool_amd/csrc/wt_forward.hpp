@@ -178,24 +178,37 @@ __device__ __forceinline__ void seed_trig(const float4& sd, float lcf, float* c,
 // Partial sums of K consecutive vertices (LC columns apart) for one wave, steps
 // [J0, J1), rotation form: (c, s) holds step J0 on entry and step J1 on exit (J1 - 1
 // when J1 == K).
-template <int K>
+// kInit: the wavefront's first wave writes its products instead of adding them, so
+// the K x 8 sums need no zero fill.
+template <int K, bool kInit>
 __device__ __forceinline__ void accumulate(float (&a)[8], float c, float s, const float4& cc, const float4& cs) {
-  a[0] = fmaf(cc.x, c, a[0]);
-  a[1] = fmaf(cc.y, c, a[1]);
-  a[2] = fmaf(cc.z, c, a[2]);
-  a[3] = fmaf(cc.w, c, a[3]);
-  a[4] = fmaf(cs.x, s, a[4]);
-  a[5] = fmaf(cs.y, s, a[5]);
-  a[6] = fmaf(cs.z, s, a[6]);
-  a[7] = fmaf(cs.w, s, a[7]);
+  if constexpr (kInit) {
+    a[0] = cc.x * c;
+    a[1] = cc.y * c;
+    a[2] = cc.z * c;
+    a[3] = cc.w * c;
+    a[4] = cs.x * s;
+    a[5] = cs.y * s;
+    a[6] = cs.z * s;
+    a[7] = cs.w * s;
+  } else {
+    a[0] = fmaf(cc.x, c, a[0]);
+    a[1] = fmaf(cc.y, c, a[1]);
+    a[2] = fmaf(cc.z, c, a[2]);
+    a[3] = fmaf(cc.w, c, a[3]);
+    a[4] = fmaf(cs.x, s, a[4]);
+    a[5] = fmaf(cs.y, s, a[5]);
+    a[6] = fmaf(cs.z, s, a[6]);
+    a[7] = fmaf(cs.w, s, a[7]);
+  }
 }
 
-template <int K, int J0, int J1>
+template <int K, int J0, int J1, bool kInit = false>
 __device__ __forceinline__ void steps_rotation(float (&acc)[K][8], float& c, float& s, const float4& cc,
                                                const float4& cs, const float4& rot) {
 #pragma unroll
   for (int j = J0; j < J1; ++j) {
-    accumulate<K>(acc[j], c, s, cc, cs);
+    accumulate<K, kInit>(acc[j], c, s, cc, cs);
     if (j + 1 < K) {
       // (c, s) <- (c cr - s sr, s cr + c sr); -sr is staged so every op is a plain FMA
       const float cn = fmaf(s, rot.z, c * rot.x);
@@ -210,12 +223,12 @@ __device__ __forceinline__ void steps_rotation(float (&acc)[K][8], float& c, flo
 
 // The same steps with the three-term recurrence; step 0 -> 1 is a rotation, which also
 // sets the previous values (cm, sm).
-template <int K, int J0, int J1>
+template <int K, int J0, int J1, bool kInit = false>
 __device__ __forceinline__ void steps_chebyshev(float (&acc)[K][8], float& c, float& s, float& cm, float& sm,
                                                 const float4& cc, const float4& cs, const float4& rot) {
 #pragma unroll
   for (int j = J0; j < J1; ++j) {
-    accumulate<K>(acc[j], c, s, cc, cs);
+    accumulate<K, kInit>(acc[j], c, s, cc, cs);
     if (j + 1 < K) {
       float cn, sn;
       if (j == 0) {
@@ -292,6 +305,86 @@ struct SegShared {
   __device__ float* xchg(int grp, int sender) { return reinterpret_cast<float*>(raw) + (grp * S + sender) * kXchgFloats; }
 };
 
+// A wavefront's place in its workgroup's segments (forward_segments).
+struct SegIds {
+  int sid, wid, lr, row_base, col_base;
+  float lcf;
+  int lane;
+};
+
+// One chunk of waves [w0, w0 + CH): the step table, then this wavefront's waves summed
+// into acc. Software pipeline over the table entries (three-term recurrence first,
+// then rotation): the seed of entry k + 1 is formed between the two halves of entry
+// k's steps (scheduling barriers keep the compiler from hoisting its dependent chain
+// back to the top of the iteration). kFirst: the first chunk with W >= S, where every
+// wavefront has a wave; its first entry writes the sums (kInit).
+template <bool kFirst, int K, int LC, int S, int CH>
+__device__ __forceinline__ void chunk_pass(const FwdParams& p, SegShared<K, LC, S, CH>& sh, const SegIds& id, int w0,
+                                           float (&acc)[K][8]) {
+  constexpr int LR = 64 / LC;
+  const int nw = p.W - w0 < CH ? p.W - w0 : CH;
+  __syncthreads();
+  const int ncheb = build_table<LC, LR, S, CH>(p, id.row_base, id.col_base, nw, id.sid, w0, sh.tab(id.wid), id.lane);
+  __syncthreads();
+  if constexpr (!kFirst) {
+    if (id.sid >= nw) return;
+  }
+  const StepEntry* tab = sh.tab(id.wid) + id.lr * (CH / S);
+  const int nmine = (nw - id.sid + S - 1) / S;
+  const float lcf = id.lcf;
+  float c, s, cm = 0.0f, sm = 0.0f;
+  seed_trig(tab[0].seed, lcf, &c, &s);
+  int k = 0;
+  if constexpr (kFirst) {
+    const StepEntry& t = tab[0];
+    const float4 rot = t.rot, cc = t.cc, cs = t.cs;
+    const float4 sd = tab[1 < nmine ? 1 : 0].seed;
+    float cn, sn;
+    if (ncheb > 0) {
+      steps_chebyshev<K, 0, K / 2, true>(acc, c, s, cm, sm, cc, cs, rot);
+      __builtin_amdgcn_sched_barrier(0);
+      seed_trig(sd, lcf, &cn, &sn);
+      __builtin_amdgcn_sched_barrier(0);
+      steps_chebyshev<K, K / 2, K, true>(acc, c, s, cm, sm, cc, cs, rot);
+    } else {
+      steps_rotation<K, 0, K / 2, true>(acc, c, s, cc, cs, rot);
+      __builtin_amdgcn_sched_barrier(0);
+      seed_trig(sd, lcf, &cn, &sn);
+      __builtin_amdgcn_sched_barrier(0);
+      steps_rotation<K, K / 2, K, true>(acc, c, s, cc, cs, rot);
+    }
+    c = cn;
+    s = sn;
+    k = 1;
+  }
+  for (; k < ncheb; ++k) {
+    const StepEntry& t = tab[k];
+    const float4 rot = t.rot, cc = t.cc, cs = t.cs;
+    const float4 sd = tab[k + 1 < nmine ? k + 1 : k].seed;
+    float cn, sn;
+    steps_chebyshev<K, 0, K / 2>(acc, c, s, cm, sm, cc, cs, rot);
+    __builtin_amdgcn_sched_barrier(0);
+    seed_trig(sd, lcf, &cn, &sn);
+    __builtin_amdgcn_sched_barrier(0);
+    steps_chebyshev<K, K / 2, K>(acc, c, s, cm, sm, cc, cs, rot);
+    c = cn;
+    s = sn;
+  }
+  for (; k < nmine; ++k) {
+    const StepEntry& t = tab[k];
+    const float4 rot = t.rot, cc = t.cc, cs = t.cs;
+    const float4 sd = tab[k + 1 < nmine ? k + 1 : k].seed;
+    float cn, sn;
+    steps_rotation<K, 0, K / 2>(acc, c, s, cc, cs, rot);
+    __builtin_amdgcn_sched_barrier(0);
+    seed_trig(sd, lcf, &cn, &sn);
+    __builtin_amdgcn_sched_barrier(0);
+    steps_rotation<K, K / 2, K>(acc, c, s, cc, cs, rot);
+    c = cn;
+    s = sn;
+  }
+}
+
 // One workgroup's segments of one grid: the whole forward program for the vertices of
 // segment group `seg_group` (kSegPerWG segments). Every thread of the block calls it.
 template <int K, int LC, int S, int kTex, int CH = kChunk>
@@ -316,53 +409,20 @@ __device__ __forceinline__ void forward_segments(const FwdParams& p, long long s
   const int col_base = seg * LC * K;
   const float lcf = (float)lc;
 
+  // The sums start from the first wave's products (kInit), so they need no zero fill;
+  // that needs every wavefront of the pair to get a wave from the first chunk (W >= S),
+  // and the rare W < S case (no wave, or one wave for two wavefronts) zeroes them first.
   float acc[K][8];
+  const SegIds ids{sid, wid, lr, row_base, col_base, lcf, lane};
+  if (p.W < S) {
 #pragma unroll
-  for (int j = 0; j < K; ++j)
+    for (int j = 0; j < K; ++j)
 #pragma unroll
-    for (int q = 0; q < 8; ++q) acc[j][q] = 0.0f;
-
-  for (int w0 = 0; w0 < p.W; w0 += CH) {
-    const int nw = p.W - w0 < CH ? p.W - w0 : CH;
-    __syncthreads();
-    const int ncheb = build_table<LC, LR, S, CH>(p, row_base, col_base, nw, sid, w0, sh.tab(wid), lane);
-    __syncthreads();
-    if (sid >= nw) continue;
-    const StepEntry* tab = sh.tab(wid) + lr * (CH / S);
-    // Software pipeline over this wavefront's table entries (three-term recurrence
-    // first, then rotation): the seed of entry k + 1 is formed between the two halves
-    // of entry k's steps (scheduling barriers keep the compiler from hoisting its
-    // dependent chain back to the top of the iteration).
-    const int nmine = (nw - sid + S - 1) / S;
-    float c, s, cm = 0.0f, sm = 0.0f;
-    seed_trig(tab[0].seed, lcf, &c, &s);
-    int k = 0;
-    for (; k < ncheb; ++k) {
-      const StepEntry& t = tab[k];
-      const float4 rot = t.rot, cc = t.cc, cs = t.cs;
-      const float4 sd = tab[k + 1 < nmine ? k + 1 : k].seed;
-      float cn, sn;
-      steps_chebyshev<K, 0, K / 2>(acc, c, s, cm, sm, cc, cs, rot);
-      __builtin_amdgcn_sched_barrier(0);
-      seed_trig(sd, lcf, &cn, &sn);
-      __builtin_amdgcn_sched_barrier(0);
-      steps_chebyshev<K, K / 2, K>(acc, c, s, cm, sm, cc, cs, rot);
-      c = cn;
-      s = sn;
-    }
-    for (; k < nmine; ++k) {
-      const StepEntry& t = tab[k];
-      const float4 rot = t.rot, cc = t.cc, cs = t.cs;
-      const float4 sd = tab[k + 1 < nmine ? k + 1 : k].seed;
-      float cn, sn;
-      steps_rotation<K, 0, K / 2>(acc, c, s, cc, cs, rot);
-      __builtin_amdgcn_sched_barrier(0);
-      seed_trig(sd, lcf, &cn, &sn);
-      __builtin_amdgcn_sched_barrier(0);
-      steps_rotation<K, K / 2, K>(acc, c, s, cc, cs, rot);
-      c = cn;
-      s = sn;
-    }
+      for (int q = 0; q < 8; ++q) acc[j][q] = 0.0f;
+    for (int w0 = 0; w0 < p.W; w0 += CH) chunk_pass<false, K, LC, S, CH>(p, sh, ids, w0, acc);
+  } else {
+    chunk_pass<true, K, LC, S, CH>(p, sh, ids, 0, acc);
+    for (int w0 = CH; w0 < p.W; w0 += CH) chunk_pass<false, K, LC, S, CH>(p, sh, ids, w0, acc);
   }
   const RowGeom g = row_geom(p, row < p.N ? row : p.N - 1);
 
