@@ -267,23 +267,26 @@ __device__ __forceinline__ void epilogue_steps(const FwdParams& p, const RowGeom
   }
 }
 
-// The S = 2 exchange of partial sums ([step][sum][lane] layout in LDS): steps
-// [J0, J0 + KS) of acc out, and the sums of the KS steps from J1 finished with the
-// partner's (fin = acc + theirs). J0 and J1 are template constants, so each wavefront's
-// code path addresses its registers directly (no per-value select on the wavefront).
-template <int K, int KS, int J0>
+// The S = 2 exchange of partial sums through LDS ([step][sum][lane], the step order
+// reversed by the second wavefront, kRev): steps [J0, J0 + KS) of acc out, and the
+// sums of the KS steps from J1 finished with the partner's (fin = acc + theirs). J0, J1
+// and kRev are template constants, so each wavefront's code path addresses its
+// registers directly; the two sides' stores also differ in address, so the compiler
+// keeps them as two paths instead of merging them into one set of stores behind a
+// select per value.
+template <int K, int KS, int J0, bool kRev>
 __device__ __forceinline__ void xchg_put(const float (&acc)[K][8], float* out, int lane) {
 #pragma unroll
   for (int j = 0; j < KS; ++j)
 #pragma unroll
-    for (int q = 0; q < 8; ++q) out[(j * 8 + q) * 64 + lane] = acc[J0 + j][q];
+    for (int q = 0; q < 8; ++q) out[((kRev ? KS - 1 - j : j) * 8 + q) * 64 + lane] = acc[J0 + j][q];
 }
-template <int K, int KS, int J1>
+template <int K, int KS, int J1, bool kRev>
 __device__ __forceinline__ void xchg_finish(const float (&acc)[K][8], const float* in, int lane, float (&fin)[KS][8]) {
 #pragma unroll
   for (int j = 0; j < KS; ++j)
 #pragma unroll
-    for (int q = 0; q < 8; ++q) fin[j][q] = acc[J1 + j][q] + in[(j * 8 + q) * 64 + lane];
+    for (int q = 0; q < 8; ++q) fin[j][q] = acc[J1 + j][q] + in[((kRev ? KS - 1 - j : j) * 8 + q) * 64 + lane];
 }
 
 // LDS of one workgroup of the forward kernels. The wave constants and step tables are
@@ -434,15 +437,15 @@ __device__ __forceinline__ void forward_segments(const FwdParams& p, long long s
     __syncthreads();
     float* mine = sh.xchg(grp, sid);
     if (sid == 0)
-      xchg_put<K, KS, KS>(acc, mine, lane);
+      xchg_put<K, KS, KS, false>(acc, mine, lane);
     else
-      xchg_put<K, KS, 0>(acc, mine, lane);
+      xchg_put<K, KS, 0, true>(acc, mine, lane);
     __syncthreads();
     const float* theirs = sh.xchg(grp, sid ^ 1);
     if (sid == 0)
-      xchg_finish<K, KS, 0>(acc, theirs, lane, fin);
+      xchg_finish<K, KS, 0, true>(acc, theirs, lane, fin);
     else
-      xchg_finish<K, KS, KS>(acc, theirs, lane, fin);
+      xchg_finish<K, KS, KS, false>(acc, theirs, lane, fin);
   } else {
 #pragma unroll
     for (int j = 0; j < KS; ++j)
