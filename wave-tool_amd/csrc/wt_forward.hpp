@@ -3,9 +3,10 @@
 //
 // Work decomposition (gfx950, wave64):
 //  * A "segment" is LR = 64/LC grid rows x (LC*K) columns. Lane l sits on row l/LC and
-//    column c0 + (l % LC); its k-th vertex is LC*k columns further. Consecutive lanes
-//    hold consecutive columns, so every store instruction writes a contiguous run of
-//    float3 (coalesced).
+//    on one of the row's LC columns c0 .. c0 + LC - 1 (for LC >= 32 and K <= 8
+//    interleaved so that lanes 16 apart hold adjacent columns); its k-th vertex is LC*k
+//    columns further. The lanes of a row hold LC consecutive columns, so every store
+//    instruction writes a contiguous run of float3 (coalesced).
 //  * S wavefronts share one segment: wavefront s of the group sums waves s, s+S, ...
 //    The S partial sums are exchanged through LDS at the end and each wavefront runs
 //    the epilogue for K/S of the K vertices. S = 2 doubles the number of resident
@@ -406,7 +407,17 @@ __device__ __forceinline__ void forward_segments(const FwdParams& p, long long s
   const int seg_id = active ? (int)seg_global : 0;
   const int rg = seg_id / segs_per_rg, seg = seg_id - rg * segs_per_rg;
   const int row_base = rg * LR;
-  const int lr = lane / LC, lc = lane - lr * LC;
+  // A wave64 VALU instruction runs as four passes of 16 lanes; with kInterleave, lanes l,
+  // l + 16, l + 32, l + 48 take neighbouring vertices (adjacent columns; for LC <= 16
+  // the same column of adjacent rows, as without it), so consecutive passes see close
+  // operand values and toggle less of the datapath (the kernel runs at the board's
+  // power limit). The lanes of a row still cover LC consecutive columns, so each store
+  // instruction writes the same contiguous runs, but a 16-lane pass of a store or a
+  // texel load spans LC / 16 times the columns: K16, with two wavefronts per SIMD and
+  // twice the epilogue per lane, measured slower with it and K <= 8 faster.
+  constexpr bool kInterleave = K <= 8 && LC >= 32;
+  const int lr = lane / LC;
+  const int lc = kInterleave ? (lane & 15) * (LC / 16) + ((lane >> 4) & (LC / 16 - 1)) : lane - lr * LC;
   const int row = row_base + lr;
   const int col0 = seg * LC * K + lc;
   const int col_base = seg * LC * K;
