@@ -314,12 +314,17 @@ __device__ __forceinline__ void adjoint_body(const FitDesc& P, int local_block, 
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int V = P.n * P.n;
   const int S = P.S;
+  // a wave64 VALU instruction runs as four passes of 16 lanes: lanes l, l + 16, l + 32,
+  // l + 48 take four consecutive points, so consecutive passes see close operand values
+  // and toggle less of the datapath (the kernel runs at the board's power limit); the
+  // wavefront still covers 64 consecutive points
+  const int pt = (wid << 6) + ((lane & 15) << 2) + (lane >> 4);
   bool active[KP];
   int vv[KP];
   double x[KP], z[KP];
 #pragma unroll
   for (int k = 0; k < KP; ++k) {
-    const int v = (local_block * KP + k) * kFitThreads + tid;
+    const int v = (local_block * KP + k) * kFitThreads + pt;
     active[k] = v < V;
     vv[k] = active[k] ? v : 0;
     x[k] = P.x0 + (double)(vv[k] % P.n) * P.spacing;
