@@ -137,9 +137,10 @@ __device__ __forceinline__ HmTap heightmap_fetch(const void* __restrict__ texels
   HmTap h;
   h.a = u - fu;
   h.b = v - fv;
-  // fmaxf/fminf return the non-NaN operand, so a NaN coordinate lands on texel -1
-  const int i0 = (int)fminf(fmaxf(fu, -1.0f), (float)(period_w - 1));
-  const int j0 = (int)fminf(fmaxf(fv, -1.0f), (float)(period_h - 1));
+  // clamp to [-1, period - 1] with one v_med3_f32; with a NaN operand it returns the
+  // min3 of the three, which is -1 here, so a NaN coordinate lands on texel -1
+  const int i0 = (int)__builtin_amdgcn_fmed3f(fu, -1.0f, (float)(period_w - 1));
+  const int j0 = (int)__builtin_amdgcn_fmed3f(fv, -1.0f, (float)(period_h - 1));
   const uint32_t c0 = wrap_lo<M>(i0, (uint32_t)W), c1 = wrap_hi<M>(i0 + 1, (uint32_t)W);
   // rows times the width in 24-bit multiplies (both factors below 2^14)
   const uint32_t r0 = __umul24(wrap_lo<M>(j0, (uint32_t)H), (uint32_t)W);
@@ -201,8 +202,8 @@ __device__ __forceinline__ VertexPending vertex_fetch(const EpilogueArgs& e, dou
       sd -= 2.0 * floor(0.5 * sd);
       td -= 2.0 * floor(0.5 * td);
     } else {
-      sd -= floor(sd);
-      td -= floor(td);
+      sd = __builtin_amdgcn_fract(sd);
+      td = __builtin_amdgcn_fract(td);
     }
     v.tap = heightmap_fetch<M>(e.texels, e.tex_w, e.tex_h, (float)sd, (float)td);
   }
