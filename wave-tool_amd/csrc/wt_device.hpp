@@ -52,7 +52,7 @@ __device__ __forceinline__ void stage_wave(const wt_gerstner_wave& g, double t, 
   o->kx = w * dx * kInvTwoPi;
   o->kz = w * dz * kInvTwoPi;
   const double ph = ((double)g.phaseConstant_phi * t + (double)g.phaseOffset_phi0) * kInvTwoPi;
-  o->c0 = ph - floor(ph);
+  o->c0 = __builtin_amdgcn_fract(ph);
   const float A = g.amplitude_A, Q = g.steepness_Q_i, wf = g.frequency_w;
   const float Dx = g.xzDirection_D[0], Dz = g.xzDirection_D[1];
   const float qa = Q * A, aw = A * wf, qaw = qa * wf;

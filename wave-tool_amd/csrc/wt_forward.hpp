@@ -134,8 +134,7 @@ __device__ __forceinline__ int build_table(const FwdParams& p, int row_base, int
       row = row < p.N ? row : p.N - 1;
       const RowGeom g = row_geom(p, row);
       const double st = fma(w.kx, g.xs, w.kz * g.zs) * (double)LC;
-      const double h = 2.0 * (st - floor(st));
-      const double f = 0.5 * (h - floor(h));  // step mod 1/2, in revolutions
+      const double f = 0.5 * __builtin_amdgcn_fract(2.0 * __builtin_amdgcn_fract(st));  // step mod 1/2, in revolutions
       cheb = cheb && f >= kChebMinRev && f <= 0.5 - kChebMinRev;
     }
   }
@@ -151,15 +150,15 @@ __device__ __forceinline__ int build_table(const FwdParams& p, int row_base, int
     WaveLds w;
     stage_wave(p.waves[w0 + i], p.time, &w);
     const double q = fma(w.kx, g.xs, w.kz * g.zs);  // revolutions per column
-    const double qf = q - floor(q);
+    const double qf = __builtin_amdgcn_fract(q);
     const double step = qf * (double)LC;
     float cr, sr;
-    sincos_rev(step - floor(step), &cr, &sr);
+    sincos_rev(__builtin_amdgcn_fract(step), &cr, &sr);
     const double ph = fma((double)col_base, q, fma(w.kx, g.xa, fma(w.kz, g.za, w.c0)));
     const double qh = floor(qf * 262144.0) * (1.0 / 262144.0);
     StepEntry t;
     t.rot = make_float4(cr, sr, -sr, 2.0f * cr);
-    t.seed = make_float4((float)(ph - floor(ph)), (float)qh, (float)(qf - qh), 0.0f);
+    t.seed = make_float4((float)__builtin_amdgcn_fract(ph), (float)qh, (float)(qf - qh), 0.0f);
     t.cc = make_float4(w.cc[0], w.cc[1], w.cc[2], w.cc[3]);
     t.cs = make_float4(w.cs[0], w.cs[1], w.cs[2], w.cs[3]);
     tab[r * kMine + pos] = t;
