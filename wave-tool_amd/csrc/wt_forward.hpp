@@ -9,7 +9,9 @@
 //    instruction writes a contiguous run of float3 (coalesced).
 //  * S wavefronts share one segment: wavefront s of the group sums waves s, s+S, ...
 //    The S partial sums are exchanged through LDS at the end and each wavefront runs
-//    the epilogue for K/S of the K vertices. S = 2 doubles the number of resident
+//    the epilogue for K/S of the K vertices (with S = 2 the second wavefront steps its
+//    vertices back from the segment's far end, so both keep their first K/2 steps and
+//    the exchange is the same code on both sides). S = 2 doubles the number of resident
 //    wavefronts for the same per-lane work, which is what lets gfx950 dual-issue VALU
 //    instructions (two waves per SIMD per quad-cycle).
 //  * Along a row the grid point is affine in the column (bilinear corners), so each
@@ -80,11 +82,14 @@ struct StepEntry {
 constexpr double kChebMinRev = 0.015942;
 
 // Build the step table of the waves this wavefront sums (i = sid, sid + S, ...) for the
-// LR rows of the segment starting at column col_base, in FP64, rounded once. Waves that
-// take the three-term recurrence come first; returns their number (wave-uniform).
+// LR rows of a segment, in FP64, rounded once. Waves that take the three-term recurrence
+// come first; returns their number (wave-uniform). seed_col: the column the entries'
+// seeds refer to (lane lc of a row seeds at seed_col + lc); rsgn: +1 for a wavefront
+// that steps towards higher columns, -1 for one that steps back from seed_col (the
+// rotation's sine negated).
 template <int LC, int LR, int S, int CH>
-__device__ __forceinline__ int build_table(const FwdParams& p, int row_base, int col_base, int nw, int sid,
-                                           int w0, StepEntry* tab, int lane) {
+__device__ __forceinline__ int build_table(const FwdParams& p, int row_base, int seed_col, float rsgn, int nw,
+                                           int sid, int w0, StepEntry* tab, int lane) {
   constexpr int kMine = CH / S;
   static_assert(kMine <= 64, "a wavefront's waves of a chunk fit the 64-lane ballot");
   const int nmine = (nw - sid + S - 1) / S;
@@ -108,7 +113,8 @@ __device__ __forceinline__ int build_table(const FwdParams& p, int row_base, int
       cheb = f >= kChebMinRev && f <= 0.5 - kChebMinRev;
       float cr, sr;
       sincos_rev(sf, &cr, &sr);
-      const double ph = fma((double)col_base, q, fma(w.kx, g.xa, fma(w.kz, g.za, w.c0)));
+      sr *= rsgn;
+      const double ph = fma((double)seed_col, q, fma(w.kx, g.xa, fma(w.kz, g.za, w.c0)));
       const double qh = floor(qf * 262144.0) * (1.0 / 262144.0);
       t.rot = make_float4(cr, sr, -sr, 2.0f * cr);
       t.seed = make_float4((float)__builtin_amdgcn_fract(ph), (float)qh, (float)(qf - qh), 0.0f);
@@ -154,7 +160,8 @@ __device__ __forceinline__ int build_table(const FwdParams& p, int row_base, int
     const double step = qf * (double)LC;
     float cr, sr;
     sincos_rev(__builtin_amdgcn_fract(step), &cr, &sr);
-    const double ph = fma((double)col_base, q, fma(w.kx, g.xa, fma(w.kz, g.za, w.c0)));
+    sr *= rsgn;
+    const double ph = fma((double)seed_col, q, fma(w.kx, g.xa, fma(w.kz, g.za, w.c0)));
     const double qh = floor(qf * 262144.0) * (1.0 / 262144.0);
     StepEntry t;
     t.rot = make_float4(cr, sr, -sr, 2.0f * cr);
@@ -166,7 +173,7 @@ __device__ __forceinline__ int build_table(const FwdParams& p, int row_base, int
   return ncheb;
 }
 
-// cos / sin of the phase of column col_base + lc from a seed entry: lc * hi is exact,
+// cos / sin of the phase of column seed_col + lc from a seed entry: lc * hi is exact,
 // so only quantities below 3 revolutions are rounded.
 __device__ __forceinline__ void seed_trig(const float4& sd, float lcf, float* c, float* s) {
   const float r = fmaf(lcf, sd.z, __builtin_amdgcn_fractf(lcf * sd.y) + sd.x);
@@ -246,46 +253,45 @@ __device__ __forceinline__ void steps_chebyshev(float (&acc)[K][8], float& c, fl
   }
 }
 
-// Epilogue of vertices [JB, JB + KS) of a lane: all texel loads first, then the per-vertex
-// math and stores (JB is a compile-time constant so the sums stay in registers).
-template <int KS, int LC, int kTex>
+// Epilogue of a lane's KS finished vertices, fin[u] at column col0 + u * dcol (dcol = LC,
+// or -LC for a wavefront that stepped back): all texel loads first, then the per-vertex
+// math and stores.
+template <int KS, int kTex>
 __device__ __forceinline__ void epilogue_steps(const FwdParams& p, const RowGeom& g, const float (&fin)[KS][8],
-                                               int col0, int row, bool active) {
+                                               int col0, int dcol, int row, bool active) {
   VertexPending pend[KS];
 #pragma unroll
   for (int u = 0; u < KS; ++u) {
-    const int col = col0 + u * LC;
+    const int col = col0 + u * dcol;
     const double x = fma((double)col, g.xs, g.xa), z = fma((double)col, g.zs, g.za);
     pend[u] = vertex_fetch<kTex>(p.ep, x, z, fin[u]);
   }
   const RowOut ro = row_out(p.ep, (size_t)(row < p.N ? row : 0) * (size_t)p.N);
 #pragma unroll
   for (int u = 0; u < KS; ++u) {
-    const int col = col0 + u * LC;
+    const int col = col0 + u * dcol;
     if (active && row < p.N && col < p.N) vertex_finish<kTex>(p.ep, pend[u], fin[u], ro, (uint32_t)col);
   }
 }
 
-// The S = 2 exchange of partial sums through LDS ([step][sum][lane], the step order
-// reversed by the second wavefront, kRev): steps [J0, J0 + KS) of acc out, and the
-// sums of the KS steps from J1 finished with the partner's (fin = acc + theirs). J0, J1
-// and kRev are template constants, so each wavefront's code path addresses its
-// registers directly; the two sides' stores also differ in address, so the compiler
-// keeps them as two paths instead of merging them into one set of stores behind a
-// select per value.
-template <int K, int KS, int J0, bool kRev>
+// The S = 2 exchange of partial sums through LDS ([slot][sum][lane]). Wavefront 0 of a
+// pair holds step j of its lane in acc[j], wavefront 1 (which steps back from the far
+// column) step K - 1 - j, so both keep acc[0, KS) and hand over acc[K - 1 - m] in slot m,
+// which is the step the partner keeps in its acc[m]: the same code on both sides, with
+// constant register indices and no branch on the wavefront.
+template <int K, int KS>
 __device__ __forceinline__ void xchg_put(const float (&acc)[K][8], float* out, int lane) {
 #pragma unroll
-  for (int j = 0; j < KS; ++j)
+  for (int m = 0; m < KS; ++m)
 #pragma unroll
-    for (int q = 0; q < 8; ++q) out[((kRev ? KS - 1 - j : j) * 8 + q) * 64 + lane] = acc[J0 + j][q];
+    for (int q = 0; q < 8; ++q) out[(m * 8 + q) * 64 + lane] = acc[K - 1 - m][q];
 }
-template <int K, int KS, int J1, bool kRev>
+template <int K, int KS>
 __device__ __forceinline__ void xchg_finish(const float (&acc)[K][8], const float* in, int lane, float (&fin)[KS][8]) {
 #pragma unroll
   for (int j = 0; j < KS; ++j)
 #pragma unroll
-    for (int q = 0; q < 8; ++q) fin[j][q] = acc[J1 + j][q] + in[((kRev ? KS - 1 - j : j) * 8 + q) * 64 + lane];
+    for (int q = 0; q < 8; ++q) fin[j][q] = acc[j][q] + in[(j * 8 + q) * 64 + lane];
 }
 
 // LDS of one workgroup of the forward kernels. The wave constants and step tables are
@@ -309,7 +315,9 @@ struct SegShared {
 
 // A wavefront's place in its workgroup's segments (forward_segments).
 struct SegIds {
-  int sid, wid, lr, row_base, col_base;
+  int sid, wid, lr, row_base;
+  int seed_col;  // the column of the lane-0 seed: the segment's first, or its last step's
+  float rsgn;    // +1 stepping towards higher columns, -1 stepping back
   float lcf;
   int lane;
 };
@@ -326,7 +334,8 @@ __device__ __forceinline__ void chunk_pass(const FwdParams& p, SegShared<K, LC, 
   constexpr int LR = 64 / LC;
   const int nw = p.W - w0 < CH ? p.W - w0 : CH;
   __syncthreads();
-  const int ncheb = build_table<LC, LR, S, CH>(p, id.row_base, id.col_base, nw, id.sid, w0, sh.tab(id.wid), id.lane);
+  const int ncheb = build_table<LC, LR, S, CH>(p, id.row_base, id.seed_col, id.rsgn, nw, id.sid, w0,
+                                               sh.tab(id.wid), id.lane);
   __syncthreads();
   if constexpr (!kFirst) {
     if (id.sid >= nw) return;
@@ -425,7 +434,10 @@ __device__ __forceinline__ void forward_segments(const FwdParams& p, long long s
   // that needs every wavefront of the pair to get a wave from the first chunk (W >= S),
   // and the rare W < S case (no wave, or one wave for two wavefronts) zeroes them first.
   float acc[K][8];
-  const SegIds ids{sid, wid, lr, row_base, col_base, lcf, lane};
+  // S = 2: the second wavefront of a pair steps its vertices back from the far column
+  // (seed there, rotation by -d), so the exchange is the same code on both sides
+  const bool back = S > 1 && sid == 1;
+  const SegIds ids{sid, wid, lr, row_base, col_base + (back ? (K - 1) * LC : 0), back ? -1.0f : 1.0f, lcf, lane};
   if (p.W < S) {
 #pragma unroll
     for (int j = 0; j < K; ++j)
@@ -438,30 +450,23 @@ __device__ __forceinline__ void forward_segments(const FwdParams& p, long long s
   }
   const RowGeom g = row_geom(p, row < p.N ? row : p.N - 1);
 
-  // the KS vertices this wavefront finishes: steps [sid * KS, sid * KS + KS)
+  // the KS vertices this wavefront finishes: its first KS steps, i.e. columns col0 + j LC
+  // for wavefront 0 and col0 + (K - 1 - j) LC for wavefront 1
   float fin[KS][8];
   if constexpr (S > 1) {
-    // S = 2: wavefront 0 keeps vertices [0, K/2), wavefront 1 keeps [K/2, K); each hands
-    // the other half of its partial sums over through LDS and adds the half it receives
+    // S = 2: each wavefront hands the partner the partial sums of the partner's steps
+    // through LDS and adds the partner's to its own
     __syncthreads();
-    float* mine = sh.xchg(grp, sid);
-    if (sid == 0)
-      xchg_put<K, KS, KS, false>(acc, mine, lane);
-    else
-      xchg_put<K, KS, 0, true>(acc, mine, lane);
+    xchg_put<K, KS>(acc, sh.xchg(grp, sid), lane);
     __syncthreads();
-    const float* theirs = sh.xchg(grp, sid ^ 1);
-    if (sid == 0)
-      xchg_finish<K, KS, 0, true>(acc, theirs, lane, fin);
-    else
-      xchg_finish<K, KS, KS, false>(acc, theirs, lane, fin);
+    xchg_finish<K, KS>(acc, sh.xchg(grp, sid ^ 1), lane, fin);
   } else {
 #pragma unroll
     for (int j = 0; j < KS; ++j)
 #pragma unroll
       for (int q = 0; q < 8; ++q) fin[j][q] = acc[j][q];
   }
-  epilogue_steps<KS, LC, kTex>(p, g, fin, col0 + sid * KS * LC, row, active);
+  epilogue_steps<KS, kTex>(p, g, fin, col0 + (back ? (K - 1) * LC : 0), back ? -LC : LC, row, active);
 }
 
 template <int K, int LC, int S, int kTex, int CH>
