@@ -223,9 +223,8 @@ def bench_batched_forward(torch, wt, scenes, frames=120, one_by_one=True):
     patches, waves_np, V = wt.batch_inputs(specs)
     vw_frame = sum(s["N"] * s["N"] * s["W"] for s in specs)
     ctx = wt.Context()
-    # segment shapes from the padded-vertex model: on this batch it beats the per-bucket
-    # autotuner (wt_autotune_batch), whose synthetic same-size batches favour the K = 4
-    # shapes more than the mixed batch does
+    # segment shapes from the padded-vertex model (wt_batch_create's default), not the
+    # per-bucket autotuner (wt_autotune_batch), which tunes on synthetic same-size batches
     batch = wt.Batch(ctx, patches)
     shapes = [batch.patch_launch_config(i) for i in range(len(patches))]
     waves = torch.from_numpy(waves_np).to(dev)

@@ -8,7 +8,7 @@ from wavetool_amd import scenes
 pytestmark = pytest.mark.gpu
 
 # the batched kernels' segment shapes (kBatchShape): (wave_split, steps, lanes_per_row, wave_chunk)
-BATCH_SHAPES = [(2, 8, 64, 64), (2, 4, 32, 64), (2, 4, 16, 64)]
+BATCH_SHAPES = [(2, 8, 64, 64), (2, 8, 32, 64), (2, 8, 16, 64)]
 
 
 def say(msg):

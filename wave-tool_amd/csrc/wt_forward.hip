@@ -27,21 +27,17 @@ long long forward_segment_groups(int N, const LaunchShape& s) {
   return (segs + per_wg - 1) / per_wg;
 }
 
-// Relative cost of a vertex-wave in the K = 4 shapes (one seed per 4 steps instead of 8,
-// and more step-table rows); 1.3 was fastest on the 512-patch workload among 1.1-2.0.
-constexpr double kBatchK4Premium = 1.3;
-
 int batch_shape_for(int N) {
-  // padded vertices per shape, K = 4 shapes weighted by their larger seed share
-  double best = 0.0;
+  // padded vertices per shape (the shapes share K, so the seeds per vertex are equal);
+  // the first, widest shape wins ties
+  long long best = 0;
   int pick = 0;
   for (int i = 0; i < kBatchShapes; ++i) {
     const LaunchShape& s = kBatchShape[i];
     const long long cols = s.lanes_per_row * s.steps, rows = 64 / s.lanes_per_row;
-    const double padded = (double)((N + cols - 1) / cols * cols) * (double)((N + rows - 1) / rows * rows);
-    const double cost = padded * (s.steps == 8 ? 1.0 : kBatchK4Premium);
-    if (i == 0 || cost < best) {
-      best = cost;
+    const long long padded = (N + cols - 1) / cols * cols * ((N + rows - 1) / rows * rows);
+    if (i == 0 || padded < best) {
+      best = padded;
       pick = i;
     }
   }

@@ -102,12 +102,12 @@ hipError_t launch_varyings(const VaryingArgs& a, hipStream_t st);
 // Segments a launch shape covers per workgroup for a grid of side N.
 long long forward_segment_groups(int N, const LaunchShape& s);
 
-// Segment shapes the batched kernel chooses from, per patch (all with S = 2): 512 x 1
-// (K = 8) for wide grids, 128 x 2 and 64 x 4 (K = 4) for narrower ones.
+// Segment shapes the batched kernel chooses from, per patch (all K = 8, S = 2): 512 x 1
+// for wide grids, 256 x 2 and 128 x 4 for narrower ones.
 constexpr int kBatchShapes = WT_BATCH_SHAPES;
-constexpr LaunchShape kBatchShape[kBatchShapes] = {{2, 8, 64, 64}, {2, 4, 32, 64}, {2, 4, 16, 64}};
-// The shape with the fewest padded vertex-waves for a grid of side N (a small premium
-// on the K = 4 shapes for their extra seeding).
+constexpr LaunchShape kBatchShape[kBatchShapes] = {{2, 8, 64, 64}, {2, 8, 32, 64}, {2, 8, 16, 64}};
+// The shape with the fewest padded vertices for a grid of side N (ties to the wider
+// shape).
 int batch_shape_for(int N);
 
 // Device projector: one thread per camera runs the reference projector in FP64.
