@@ -129,11 +129,17 @@ __device__ __forceinline__ int build_table(const FwdParams& p, int row_base, int
     }
     return ncheb;
   }
-  // which waves qualify for the recurrence on every row of the segment
-  bool cheb = lane < nmine;
-  if (cheb) {
-    WaveLds w;
-    stage_wave(p.waves[w0 + sid + lane * S], p.time, &w);
+  // several rows per segment: the kParts lanes m, m + kMine, ... stage wave m once and
+  // check on every row of the segment whether it qualifies for the recurrence; after the
+  // ballot each of them forms the wave's entries of every kParts-th row
+  constexpr int kParts = 64 / kMine;
+  static_assert(LR == 1 || (kParts * kMine == 64 && LR % kParts == 0), "a chunk's waves split the lanes and rows evenly");
+  const int m = lane & (kMine - 1), part = lane / kMine;
+  const bool mine = m < nmine;
+  bool cheb = mine;
+  WaveLds w;
+  if (mine) {
+    stage_wave(p.waves[w0 + sid + m * S], p.time, &w);
 #pragma unroll
     for (int r = 0; r < LR; ++r) {
       int row = row_base + r;
@@ -144,17 +150,19 @@ __device__ __forceinline__ int build_table(const FwdParams& p, int row_base, int
       cheb = cheb && f >= kChebMinRev && f <= 0.5 - kChebMinRev;
     }
   }
-  const uint64_t cmask = __ballot(cheb) & valid;
+  const uint64_t cmask = __ballot(cheb) & valid;  // valid holds lanes of the first part only
   const int ncheb = __popcll(cmask);
-  for (int e = lane; e < LR * nmine; e += 64) {
-    const int r = e / nmine, m = e - r * nmine, i = sid + m * S;
-    const uint64_t mb = (1ull << m) - 1ull;
-    const int pos = ((cmask >> m) & 1ull) ? __popcll(cmask & mb) : ncheb + __popcll(~cmask & valid & mb);
+  if (!mine) return ncheb;
+  const uint64_t mb = (1ull << m) - 1ull;
+  const int pos = cheb ? __popcll(cmask & mb) : ncheb + __popcll(~cmask & valid & mb);
+  // rows one at a time (unrolled, the entries' FP64 temporaries of several rows would
+  // be live together)
+#pragma unroll 1
+  for (int rr = 0; rr < LR / kParts; ++rr) {
+    const int r = part + rr * kParts;
     int row = row_base + r;
     row = row < p.N ? row : p.N - 1;
     const RowGeom g = row_geom(p, row);
-    WaveLds w;
-    stage_wave(p.waves[w0 + i], p.time, &w);
     const double q = fma(w.kx, g.xs, w.kz * g.zs);  // revolutions per column
     const double qf = __builtin_amdgcn_fract(q);
     const double step = qf * (double)LC;
